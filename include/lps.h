@@ -1,0 +1,161 @@
+/*
+ * include/lps.h -- C ABI of the MI355X SPH/PCISPH step library (liblps.so).
+ *
+ * The library replaces the step path of the reference's LiquidPhysics solver:
+ *   PhysicSolver::update(dt)          LiquidPhysics/src/PhysicSolver.cpp:12-21
+ * whose phases branch on ParallelizationType {NONE, CPU, GPU}
+ *   (LiquidPhysics/include/SPHSolver.h:19-23) and reach the OpenCL kernels in
+ *   LiquidPhysics/cl_kernels/SPHKernels.cl through ParallelComputationInterface
+ *   (LiquidPhysics/include/Parallelization/ParallelComputationInterface.h:35-58).
+ * A solver adapter (a fourth ParallelizationType) calls these entry points in
+ * place of the OpenCL launch sequence; see INTEGRATION.md for the binding.
+ *
+ * Conventions: plain C types, host arrays owned by the caller and copied in
+ * or out; device memory owned by the context.  Every function returns an
+ * lps_status.  One HIP stream per context; a context is thread-compatible but
+ * not thread-safe (the reference solver is single-threaded too,
+ * LiquidPhysics/src/Parallelization/OpenCLInterface.cpp:121,131).
+ * Particle state is kept in the reference's sorted order between steps: each
+ * step sorts by grid cell, stable with respect to the incoming order, exactly
+ * as the OpenCL path does (SPHSolver.cpp:1049-1119, readback order 412-422).
+ */
+#ifndef LPS_H
+#define LPS_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LPS_ABI_VERSION 1u
+#define LPS_TABLE_SIZE 1001 /* SPHKernel subdivision 1000 + 1 (Kernels/SPHKernel.cpp:4-7,124) */
+
+typedef enum lps_status {
+    LPS_OK = 0,
+    LPS_ERR_INVALID = 1,       /* bad argument, size or ABI version */
+    LPS_ERR_NO_DEVICE = 2,     /* no usable gfx950 device (caller falls back) */
+    LPS_ERR_OOM = 3,           /* device allocation failed */
+    LPS_ERR_NONFINITE = 4,     /* NaN/inf in the particle state or a density error */
+    LPS_ERR_GRID_OVERFLOW = 5, /* bounding-box cell count beyond the supported maximum */
+    LPS_ERR_HIP = 6,           /* other HIP runtime error */
+    LPS_ERR_STATE = 7          /* call out of order (e.g. step before particles) */
+} lps_status;
+
+typedef enum lps_method { LPS_METHOD_SPH = 0, LPS_METHOD_PCISPH = 1 } lps_method;
+
+typedef enum lps_warm_start {
+    LPS_WARM_NONE = 0,    /* pciInit: p = 0, fp = 0 (SPHKernels.cl:661-672) */
+    LPS_WARM_LEARNED = 1  /* MLP-seeded pressure, falls back to NONE when it does not help */
+} lps_warm_start;
+
+/* Fluid and solver parameters.  Field meanings follow the reference setters
+ * (SPHSolver.h:57-69, PCISPHSolver.h:19-20); derived quantities (kernel radius,
+ * lookup tables, particle mass, PCISPH delta) follow SPHSolver.cpp:480-523 and
+ * PCISPHSolver.cpp:286-328. */
+typedef struct lps_params {
+    uint32_t abi_version;            /* LPS_ABI_VERSION */
+    int32_t method;                  /* lps_method */
+    float gravity[3];                /* setGravity */
+    float particle_radius;           /* setParticleRadius */
+    float kernel_radius_factor;      /* setKernelRadiusFactor: h = factor * r_p */
+    float rest_density;              /* setRestDensity */
+    float stiffness;                 /* setPressureStiffnessCoefficient (SPH) */
+    float negative_pressure_factor;  /* setNegativePressureFactor */
+    float viscosity;                 /* setViscosityCoefficient */
+    float surface_tension;           /* setSurfaceTensionCoefficient */
+    float surface_tension_threshold; /* setSurfaceTensionThreshold */
+    float restitution;               /* setRestitutionCoefficient */
+    float friction;                  /* setFrictionCoefficient */
+    float mass_override;             /* <= 0: the reference lattice rule */
+    int32_t min_iterations;          /* PCISPH corrections always run (>= 1) */
+    int32_t max_iterations;          /* PCISPH cap */
+    float max_density_error_ratio;   /* setMaxDensityErrorRatio: stop when max|e|/rho0 < this */
+    int32_t warm_start;              /* lps_warm_start */
+} lps_params;
+
+/* Static colliders (StaticCollisionBox / StaticCollisionSphere, Collision/*.h),
+ * plus a plane that the reference lacks. */
+typedef struct lps_box { float center[3]; float half[3]; int32_t boundary; } lps_box;
+typedef struct lps_sphere { float center[3]; float radius; int32_t boundary; } lps_sphere;
+typedef struct lps_plane { float normal[3]; float offset; } lps_plane; /* fluid side: n.x >= offset */
+
+typedef struct lps_step_stats {
+    int32_t iterations;              /* PCISPH corrections run this step (0 for SPH) */
+    float max_density_error_ratio;   /* max_i |e_i| / rho0 after the last correction */
+    int32_t warm_start_used;         /* 1 if the learned seed was applied */
+    int32_t status;                  /* lps_status of this step */
+} lps_step_stats;
+
+typedef struct lps_grid_info {
+    float bbox_min[3];
+    float bbox_max[3];
+    float grid_offset[3];
+    float grid_spacing;
+    uint32_t grid_size[3];
+    uint32_t cell_count;
+} lps_grid_info;
+
+typedef struct lps_ctx lps_ctx;
+
+/* Context lifetime.  Replaces OpenCLInterface::initialize/reinitContext
+ * (OpenCLInterface.cpp:65-135) and the buffer set-up of
+ * SPHSolver::reinitParallelContext (SPHSolver.cpp:713-786). */
+int lps_create(const lps_params* params, lps_ctx** out);
+int lps_destroy(lps_ctx* ctx);
+int lps_device_info(char* name, size_t cap, int32_t* cu_count);
+
+/* Parameters and derived quantities (SPHSolver setters, SPHSolver.cpp:628-710). */
+int lps_set_params(lps_ctx* ctx, const lps_params* params);
+int lps_get_derived(lps_ctx* ctx, float* kernel_radius, float* division_step, float* mass,
+                    float* tables /* 5 x LPS_TABLE_SIZE: W, W', W'' (poly6), spiky', visc'' */);
+int lps_get_delta(lps_ctx* ctx, float dt, float* delta);
+
+/* Scene.  lps_set_particles replaces the upload of SPHSolver::initParallelBuffers
+ * (SPHSolver.cpp:788-886); arrays are xyz-interleaved [3n]; vhalf and first may
+ * be NULL (zero half velocity, first step set). */
+int lps_set_particles(lps_ctx* ctx, const float* x, const float* v, const float* vhalf,
+                      const uint8_t* first, uint32_t n);
+/* Readback of SPHSolver::onEndUpdate (SPHSolver.cpp:398-428), in sorted order.
+ * Any pointer may be NULL; id returns each particle's index at lps_set_particles. */
+int lps_get_particles(lps_ctx* ctx, float* x, float* v, float* vhalf, uint8_t* first, uint32_t* id,
+                      uint32_t n);
+int lps_get_count(lps_ctx* ctx, uint32_t* n);
+/* Collider upload of SPHSolver::initParallelBuffers (SPHSolver.cpp:932-1003). */
+int lps_set_colliders(lps_ctx* ctx, const lps_box* boxes, uint32_t nbox, const lps_sphere* spheres,
+                      uint32_t nsphere, const lps_plane* planes, uint32_t nplane);
+
+/* One or more steps of PhysicSolver::update (PhysicSolver.cpp:12-21) on the device.
+ * stats may be NULL or hold nsteps entries.  Returns the first failing status. */
+int lps_step(lps_ctx* ctx, float dt, uint32_t nsteps, lps_step_stats* stats);
+
+/* Parity hooks: the grid and fields of the last step (sorted order).
+ * keys/perm [n]; cell_start/cell_end [cap] follow buildCellList's rule
+ * (SPHKernels.cl:148-186, end rule :230).  Any pointer may be NULL. */
+int lps_get_grid(lps_ctx* ctx, lps_grid_info* info, uint32_t* keys, uint32_t* perm, uint32_t* cell_start,
+                 uint32_t* cell_end, uint32_t cap);
+int lps_get_fields(lps_ctx* ctx, float* rho, float* pressure, float* force /* [3n] */,
+                   float* rho_pred);
+
+/* Learned warm start (PCISPH): 16 -> 64 -> 64 -> 1 MLP, BF16 weights. */
+#define LPS_MLP_IN 16
+#define LPS_MLP_HIDDEN 64
+#define LPS_MLP_PARAMS (LPS_MLP_IN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + 1)
+int lps_mlp_set_weights(lps_ctx* ctx, const float* params /* LPS_MLP_PARAMS, fp32 master copy */,
+                        const float* feature_scale /* [2*LPS_MLP_IN]: mean, inv_std */, float target_scale);
+int lps_mlp_get_weights(lps_ctx* ctx, float* params, float* feature_scale, float* target_scale);
+/* Dump (features -> converged pressure) pairs of the last PCISPH step. */
+int lps_dump_pairs(lps_ctx* ctx, float* features /* [n*16] */, float* target /* [n] */, uint32_t n);
+
+/* Kernel timing (HIP events on the context stream), for measurement only. */
+int lps_profile_enable(lps_ctx* ctx, int32_t on);
+int lps_profile_query(lps_ctx* ctx, const char* kernel, uint32_t* launches, double* total_ms);
+int lps_profile_reset(lps_ctx* ctx);
+/* The context's HIP stream (hipStream_t), for callers that record their own events. */
+int lps_get_stream(lps_ctx* ctx, void** stream);
+const char* lps_status_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LPS_H */

@@ -1,0 +1,184 @@
+"""Seeded dam-break scenes for the benchmark workloads.
+
+The reference builds its scenes in the Qt scenarios with
+SPHParticleEmitter::spawnCube (LiquidPhysics/src/Particles/SPHParticleEmitter.cpp:13-43,
+used by LiquidSimulation/src/Scenarios/DamBreakScenario.cpp:9-35).  The workloads
+here place a block of nx*ny*nz particles on the lattice x = s*(i+1) (FP32,
+i = 0..nx-1, likewise y and z) in the min corner of a BOUNDARY box, in the
+emitter's loop order (x outer, y, z inner), and add i.i.d. uniform jitter.
+
+The lattice step is s = 1.6 * r_p, the step of the reference's mass lattice
+(SPHSolver.cpp:485), so an interior particle of the unjittered block has the
+rest density by construction of the particle mass.
+
+Jitter generator (documented so any consumer can regenerate it): splitmix64
+with state seed, one output per component in (particle, axis) order;
+u = (z >> 40) * 2^-24 in FP32 (exact), jitter = amp * (2u - 1) in FP32.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from .abi import Box, Colliders, Params, Plane, Sphere, METHOD_PCISPH, METHOD_SPH
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """count outputs of splitmix64 starting from state `seed` (uint64 wraparound)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, count: int) -> np.ndarray:
+    z = splitmix64(seed, count)
+    return (z >> np.uint64(40)).astype(np.float32) * np.float32(2.0 ** -24)
+
+
+def jitter(seed: int, count: int, amp: float) -> np.ndarray:
+    u = uniform01(seed, count)
+    return np.float32(amp) * (np.float32(2.0) * u - np.float32(1.0))
+
+
+def block_lattice(nx: int, ny: int, nz: int, s: float, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
+    """[n,3] FP32 lattice origin + s*(i+1), x-outer / z-inner order."""
+    s32 = np.float32(s)
+    ix = (np.arange(nx, dtype=np.float32) + np.float32(1.0)) * s32 + np.float32(origin[0])
+    iy = (np.arange(ny, dtype=np.float32) + np.float32(1.0)) * s32 + np.float32(origin[1])
+    iz = (np.arange(nz, dtype=np.float32) + np.float32(1.0)) * s32 + np.float32(origin[2])
+    X, Y, Z = np.meshgrid(ix, iy, iz, indexing="ij")
+    return np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(np.float32)
+
+
+@dataclass
+class Scene:
+    name: str
+    params: Params
+    colliders: Colliders
+    x: np.ndarray               # [n,3] float32
+    v: np.ndarray               # [n,3] float32
+    dt: float
+    steps: int
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.x.shape[0])
+
+
+def _fluid_params(s: float, method: int, **kw) -> Params:
+    # r_p = s / 1.6 so the reference's mass lattice step 1.6 r_p equals s;
+    # h = 2 s = 3.2 r_p.  Fluid constants: rest density 1000, stiffness 3.0,
+    # viscosity 0.01; surface tension 0.0728 / threshold 7.065, restitution 0.5,
+    # friction 1.0 are the reference constructor defaults (SPHSolver.cpp:11-21).
+    p = Params(
+        method=method,
+        gravity=(0.0, -9.81, 0.0),
+        particle_radius=float(np.float32(s) / np.float32(1.6)),
+        kernel_radius_factor=3.2,
+        rest_density=1000.0,
+        stiffness=3.0,
+        negative_pressure_factor=0.0,
+        viscosity=0.01,
+        surface_tension=0.0728,
+        surface_tension_threshold=7.065,
+        restitution=0.5,
+        friction=1.0,
+        min_iterations=3,
+        max_iterations=50,
+        max_density_error_ratio=0.01,
+    )
+    return p.replace(**kw) if kw else p
+
+
+def dam_break(nx: int, ny: int, nz: int, s: float, box=(1.6, 1.0, 0.8), jitter_amp: float = 0.0,
+              seed: int = 0, method: int = METHOD_PCISPH, dt: float = 0.001, steps: int = 100,
+              name: Optional[str] = None, **param_kw) -> Scene:
+    x = block_lattice(nx, ny, nz, s)
+    if jitter_amp > 0.0:
+        x = (x + jitter(seed, x.size, jitter_amp).reshape(x.shape)).astype(np.float32)
+    bx, by, bz = (float(b) for b in box)
+    col = Colliders(boxes=[Box(center=(bx / 2, by / 2, bz / 2), half=(bx / 2, by / 2, bz / 2), boundary=True)])
+    p = _fluid_params(s, method, **param_kw)
+    return Scene(name=name or f"dam_break_{nx}x{ny}x{nz}", params=p, colliders=col, x=x,
+                 v=np.zeros_like(x), dt=dt, steps=steps,
+                 meta={"nx": nx, "ny": ny, "nz": nz, "spacing": s, "box": box, "jitter": jitter_amp,
+                       "seed": seed})
+
+
+def sph_dam_break_16k(seed: int = 1) -> Scene:
+    """Plain SPH dam break, 16,000 particles: 20x40x20 at 0.02 m in a 1.6x1.0x0.8 m box,
+    dt 0.002 s, 500 steps, +-1e-4 m jitter."""
+    return dam_break(20, 40, 20, 0.02, jitter_amp=1e-4, seed=seed, method=METHOD_SPH, dt=0.002,
+                     steps=500, name="sph_dam_break_16k")
+
+
+def pcisph_dam_break_524k(seed: int = 2) -> Scene:
+    """PCISPH dam break, 524,288 particles: 64x128x64 at 0.00625 m in the same box,
+    dt 0.001 s, 2000 steps, +-1e-5 m jitter, tolerance 1 %, 3..50 corrections."""
+    return dam_break(64, 128, 64, 0.00625, jitter_amp=1e-5, seed=seed, method=METHOD_PCISPH,
+                     dt=0.001, steps=2000, name="pcisph_dam_break_524k")
+
+
+def obstacle_spheres(seed: int, count: int, block_max: float, box=(2.0, 1.0, 1.0)) -> List[Sphere]:
+    """`count` non-overlapping OBSTACLE spheres, radius U[0.05, 0.15], centres
+    drawn by rejection from splitmix64(seed) in x in [block_max + r + 0.02, 1.3],
+    y in [r, 0.45], z in [r, box_z - r]."""
+    out: List[Sphere] = []
+    u = uniform01(seed, 4 * 1000)
+    k = 0
+    while len(out) < count and k + 4 <= u.size:
+        r = 0.05 + 0.10 * float(u[k])
+        lo = block_max + r + 0.02
+        cx = lo + (1.3 - lo) * float(u[k + 1])
+        cy = r + (0.45 - r) * float(u[k + 2])
+        cz = r + (box[2] - 2 * r) * float(u[k + 3])
+        k += 4
+        ok = all(math.dist((cx, cy, cz), sp.center) > r + sp.radius + 0.02 for sp in out)
+        if ok:
+            out.append(Sphere(center=(cx, cy, cz), radius=r, boundary=False))
+    return out
+
+
+def ramp_plane(angle_deg: float = 20.0, x0: float = 1.3) -> Plane:
+    """A plane tilted `angle_deg` from the floor, rising towards +x from (x0, 0, z):
+    normal (-sin a, cos a, 0), fluid side n.x >= offset."""
+    a = math.radians(angle_deg)
+    n = (-math.sin(a), math.cos(a), 0.0)
+    return Plane(normal=n, offset=n[0] * x0)
+
+
+def pcisph_obstacles_2m(seed: int = 3) -> Scene:
+    """PCISPH with static obstacles, 2,097,152 particles: 128^3 at 0.004 m in a
+    2.0x1.0x1.0 m box, 3 obstacle spheres and a 20 degree plane, dt 0.0005 s, 1000 steps."""
+    sc = dam_break(128, 128, 128, 0.004, box=(2.0, 1.0, 1.0), jitter_amp=1e-5, seed=seed,
+                   method=METHOD_PCISPH, dt=0.0005, steps=1000, name="pcisph_obstacles_2m")
+    block_max = 0.004 * 129
+    sc.colliders.spheres = obstacle_spheres(seed + 1000, 3, block_max, box=(2.0, 1.0, 1.0))
+    sc.colliders.planes = [ramp_plane(20.0, 1.3)]
+    return sc
+
+
+def small_block(nx=8, ny=8, nz=8, s=0.02, jitter_amp=1e-4, seed=5, method=METHOD_PCISPH, dt=0.002,
+                box=None, **kw) -> Scene:
+    """A small dam break for unit and parity tests (box twice the block's extent)."""
+    if box is None:
+        box = (4 * s * (nx + 1), 2 * s * (ny + 1), 2 * s * (nz + 1))
+    return dam_break(nx, ny, nz, s, box=box, jitter_amp=jitter_amp, seed=seed, method=method, dt=dt,
+                     steps=10, name=f"small_{nx}x{ny}x{nz}", **kw)
+
+
+WORKLOADS = {
+    "sph_dam_break_16k": sph_dam_break_16k,
+    "pcisph_dam_break_524k": pcisph_dam_break_524k,
+    "pcisph_obstacles_2m": pcisph_obstacles_2m,
+}
