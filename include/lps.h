@@ -74,7 +74,7 @@ typedef struct lps_params {
     int32_t warm_start;              /* lps_warm_start */
 } lps_params;
 
-/* Static colliders (StaticCollisionBox / StaticCollisionSphere, Collision/*.h),
+/* Static colliders (StaticCollisionBox / StaticCollisionSphere, include/Collision/),
  * plus a plane that the reference lacks. */
 typedef struct lps_box { float center[3]; float half[3]; int32_t boundary; } lps_box;
 typedef struct lps_sphere { float center[3]; float radius; int32_t boundary; } lps_sphere;

@@ -1,0 +1,784 @@
+// lps_api.hip -- C ABI (include/lps.h) and the host orchestration of one step.
+//
+// The reference's OpenCL branch drives a step from the host with blocking
+// transfers in the middle (host bounding box SPHSolver.cpp:1011-1029, host
+// radix scan :1066-1078, per-step buffer re-creation :1123-1125,
+// PCISPHSolver.cpp:271-283, readback :398-428).  Here the whole step stays on
+// the device: bounding box, grid parameters, sort, cell table and the PCISPH
+// convergence test are kernels, and the host only enqueues.  The PCISPH loop
+// keeps one correction queued ahead of the convergence result it waits for;
+// corrections issued after convergence see the device "done" flag and return
+// immediately.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lps_common.h"
+#include "lps_kernels.h"
+
+namespace {
+
+struct ProfRec {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct lps_ctx {
+    lps_params params{};
+    Derived der{};
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    uint32_t n = 0;         // particles
+    uint32_t cap_n = 0;     // allocated particle capacity
+    uint32_t cell_cap = 0;  // allocated cell-table entries (plus one sentinel)
+    uint32_t passes = 0;    // radix passes for cell_cap
+    int cur = 0;            // which state buffer holds the sorted state
+    uint32_t parity = 0;    // bbox accumulator read by the next step
+    uint64_t steps = 0;
+    bool have_step = false; // a step has produced grid/fields
+
+    // device state
+    float4 *pos[2] = {}, *vel[2] = {}, *vh[2] = {};
+    uint8_t* first[2] = {};
+    uint32_t* id[2] = {};
+    float* pprev[2] = {};
+    uint32_t *keysA = nullptr, *valsA = nullptr, *keysB = nullptr, *valsB = nullptr;
+    uint32_t *skeys = nullptr, *perm = nullptr;
+    uint32_t* hist = nullptr;
+    float4 *posrho = nullptr, *force = nullptr, *fp = nullptr, *xpred[2] = {};
+    float *press = nullptr, *q = nullptr, *rho_pred = nullptr;
+    uint32_t *run_end = nullptr, *cell_lb = nullptr, *block_max = nullptr;
+    float* tables = nullptr;
+    lps_box* boxes = nullptr;
+    lps_sphere* spheres = nullptr;
+    lps_plane* planes = nullptr;
+    uint32_t nbox = 0, nsphere = 0, nplane = 0;
+    std::vector<lps_box> h_boxes;
+    DevGrid* grid = nullptr;
+
+    // host
+    uint32_t* flags = nullptr;  // pinned, coherent: 4 words per iteration slot
+    DevGrid* h_grid = nullptr;  // pinned copy
+    hipEvent_t iter_ev[LPS_MAX_ITER_SLOTS] = {};
+    int last_fp_valid = 0;      // PCISPH fields available
+
+    // learned warm start: fp32 master weights, feature normalisation, target scale
+    std::vector<float> mlp_w;
+    std::vector<float> mlp_fscale;
+    float mlp_tscale = 1.f;
+    bool mlp_loaded = false;
+
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    std::vector<ProfRec> pending;
+    std::map<std::string, std::pair<uint32_t, double>> prof_acc;
+};
+
+namespace {
+
+int hip_status(hipError_t e) {
+    if (e == hipSuccess) return LPS_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return LPS_ERR_OOM;
+    return LPS_ERR_HIP;
+}
+
+#define HIPC(expr)                              \
+    do {                                        \
+        hipError_t e_ = (expr);                 \
+        if (e_ != hipSuccess) return hip_status(e_); \
+    } while (0)
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    if (count == 0) count = 1;
+    HIPC(hipMalloc((void**)p, sizeof(T) * count));
+    return LPS_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+hipEvent_t prof_event(lps_ctx* c) {
+    if (c->ev_next == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_next++];
+}
+
+struct Timed {
+    lps_ctx* c;
+    const char* name;
+    hipEvent_t a = nullptr;
+    Timed(lps_ctx* ctx, const char* nm) : c(ctx), name(nm) {
+        if (c->prof) {
+            a = prof_event(c);
+            if (a) (void)hipEventRecord(a, c->stream);
+        }
+    }
+    ~Timed() {
+        if (c->prof && a) {
+            hipEvent_t b = prof_event(c);
+            if (b) {
+                (void)hipEventRecord(b, c->stream);
+                c->pending.push_back({name, a, b});
+            }
+        }
+    }
+};
+
+void prof_collect(lps_ctx* c) {
+    if (c->pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (const ProfRec& r : c->pending) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        auto& acc = c->prof_acc[r.name];
+        acc.first += 1;
+        acc.second += (double)ms;
+    }
+    c->pending.clear();
+    c->ev_next = 0;
+}
+
+SimConst make_const(const lps_ctx* c, float dt, float delta) {
+    SimConst k{};
+    for (int a = 0; a < 3; a++) k.gravity[a] = c->params.gravity[a];
+    k.particle_radius = c->params.particle_radius;
+    k.h = c->der.h;
+    k.div = c->der.div;
+    k.s_max = c->der.s_max;
+    k.mass = c->der.mass;
+    k.rest_density = c->params.rest_density;
+    k.stiffness = c->params.stiffness;
+    k.npf = c->params.negative_pressure_factor;
+    k.viscosity = c->params.viscosity;
+    k.surface_tension = c->params.surface_tension;
+    k.st_threshold = c->params.surface_tension_threshold;
+    k.restitution = c->params.restitution;
+    k.friction = c->params.friction;
+    k.dt = dt;
+    k.delta = delta;
+    k.n = c->n;
+    k.nbox = c->nbox;
+    k.nsphere = c->nsphere;
+    k.nplane = c->nplane;
+    k.cell_cap = c->cell_cap;
+    return k;
+}
+
+int check_params(const lps_params* p) {
+    if (!p || p->abi_version != LPS_ABI_VERSION) return LPS_ERR_INVALID;
+    if (p->method != LPS_METHOD_SPH && p->method != LPS_METHOD_PCISPH) return LPS_ERR_INVALID;
+    if (!(p->particle_radius > 0.f) || !(p->kernel_radius_factor > 0.f) || !(p->rest_density > 0.f))
+        return LPS_ERR_INVALID;
+    if (p->method == LPS_METHOD_PCISPH && (p->max_iterations < 1 || p->max_iterations > 10000)) return LPS_ERR_INVALID;
+    return LPS_OK;
+}
+
+int upload_tables(lps_ctx* c) {
+    if (!c->tables) HIPC(hipMalloc((void**)&c->tables, sizeof(float) * 5 * LPS_TABLE_SIZE));
+    HIPC(hipMemcpyAsync(c->tables, c->der.tables, sizeof(float) * 5 * LPS_TABLE_SIZE, hipMemcpyHostToDevice,
+                        c->stream));
+    return LPS_OK;
+}
+
+// Cell-count bound: particles end every step inside each BOUNDARY box shrunk
+// by the particle radius, so that box bounds the grid; the first step's grid
+// comes from the initial positions.
+uint32_t cell_bound(const lps_ctx* c, const float* x, uint32_t n) {
+    const float h = c->der.h;
+    uint64_t init = 1, bound = 0;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; i++)
+        for (int a = 0; a < 3; a++) {
+            if (!std::isfinite(x[3 * i + a])) continue;
+            mn[a] = std::min(mn[a], x[3 * i + a]);
+            mx[a] = std::max(mx[a], x[3 * i + a]);
+        }
+    for (int a = 0; a < 3; a++) {
+        const double ex = mx[a] >= mn[a] ? ((double)mx[a] - (double)mn[a]) / (double)h : 0.0;
+        init *= (uint64_t)std::min(ex, 4.0e9) + 2;
+    }
+    init = std::min<uint64_t>(init, 0x7FFFFFF0ull);
+    for (const lps_box& b : c->h_boxes) {
+        if (!b.boundary) continue;
+        uint64_t v = 1;
+        for (int a = 0; a < 3; a++) v *= (uint64_t)(2.f * std::max(b.half[a] - c->params.particle_radius, 0.f) / h) + 2;
+        bound = bound ? std::min(bound, v) : v;
+    }
+    uint64_t cap = bound ? std::max(bound, init) : init * 8;
+    cap += cap / 8 + 64;
+    return (uint32_t)std::min<uint64_t>(cap, 0x7FFFFFF0ull);
+}
+
+int ensure_cells(lps_ctx* c, uint32_t cap) {
+    if (cap <= c->cell_cap && c->cell_lb) return LPS_OK;
+    int st;
+    if ((st = dalloc(&c->cell_lb, (size_t)cap + 1))) return st;
+    if ((st = dalloc(&c->run_end, (size_t)cap + 1))) return st;
+    if ((st = dalloc(&c->block_max, lps_dev::cell_blocks(cap) + 1))) return st;
+    HIPC(hipMemsetAsync(c->run_end, 0, sizeof(uint32_t) * ((size_t)cap + 1), c->stream));
+    c->cell_cap = cap;
+    const uint32_t bits = cap > 1 ? 32u - (uint32_t)__builtin_clz(cap - 1) : 1u;
+    c->passes = (bits + 7) / 8;
+    return LPS_OK;
+}
+
+int ensure_particles(lps_ctx* c, uint32_t n) {
+    if (n <= c->cap_n && c->pos[0]) return LPS_OK;
+    int st;
+    for (int b = 0; b < 2; b++) {
+        if ((st = dalloc(&c->pos[b], n))) return st;
+        if ((st = dalloc(&c->vel[b], n))) return st;
+        if ((st = dalloc(&c->vh[b], n))) return st;
+        if ((st = dalloc(&c->first[b], n))) return st;
+        if ((st = dalloc(&c->id[b], n))) return st;
+        if ((st = dalloc(&c->pprev[b], n))) return st;
+        if ((st = dalloc(&c->xpred[b], n))) return st;
+    }
+    if ((st = dalloc(&c->keysA, n))) return st;
+    if ((st = dalloc(&c->valsA, n))) return st;
+    if ((st = dalloc(&c->keysB, n))) return st;
+    if ((st = dalloc(&c->valsB, n))) return st;
+    if ((st = dalloc(&c->skeys, n))) return st;
+    if ((st = dalloc(&c->perm, n))) return st;
+    if ((st = dalloc(&c->hist, (size_t)256 * lps_dev::radix_tiles(n)))) return st;
+    if ((st = dalloc(&c->posrho, n))) return st;
+    if ((st = dalloc(&c->force, n))) return st;
+    if ((st = dalloc(&c->fp, n))) return st;
+    if ((st = dalloc(&c->press, n))) return st;
+    if ((st = dalloc(&c->q, n))) return st;
+    if ((st = dalloc(&c->rho_pred, n))) return st;
+    c->cap_n = n;
+    return LPS_OK;
+}
+
+void free_all(lps_ctx* c) {
+    for (int b = 0; b < 2; b++) {
+        dfree(c->pos[b]); dfree(c->vel[b]); dfree(c->vh[b]); dfree(c->first[b]);
+        dfree(c->id[b]); dfree(c->pprev[b]); dfree(c->xpred[b]);
+    }
+    dfree(c->keysA); dfree(c->valsA); dfree(c->keysB); dfree(c->valsB);
+    dfree(c->skeys); dfree(c->perm); dfree(c->hist);
+    dfree(c->posrho); dfree(c->force); dfree(c->fp); dfree(c->press); dfree(c->q); dfree(c->rho_pred);
+    dfree(c->run_end); dfree(c->cell_lb); dfree(c->block_max);
+    dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
+    if (c->flags) (void)hipHostFree(c->flags);
+    if (c->h_grid) (void)hipHostFree(c->h_grid);
+    c->flags = nullptr;
+    c->h_grid = nullptr;
+}
+
+lps_dev::PhysArgs phys_args(lps_ctx* c, const SimConst& k) {
+    lps_dev::PhysArgs a;
+    const int s = c->cur;
+    a.pos = c->pos[s];
+    a.vel = c->vel[s];
+    a.vh = c->vh[s];
+    a.first = c->first[s];
+    a.posrho = c->posrho;
+    a.force = c->force;
+    a.g = c->grid;
+    a.cell_lb = c->cell_lb;
+    a.tables = c->tables;
+    a.boxes = c->boxes;
+    a.spheres = c->spheres;
+    a.planes = c->planes;
+    a.c = k;
+    return a;
+}
+
+// Grid build of one step: keys, sort, reorder (state flips to the other
+// buffer), cell table.
+void enqueue_grid(lps_ctx* c) {
+    const hipStream_t s = c->stream;
+    const uint32_t n = c->n;
+    {
+        Timed t(c, "grid_setup");
+        lps_dev::launch_grid_setup(s, c->grid, c->parity, c->der.h, c->cell_cap);
+    }
+    {
+        Timed t(c, "keys");
+        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, c->keysA, c->valsA);
+    }
+    for (uint32_t p = 0; p < c->passes; p++) {
+        Timed t(c, "radix_pass");
+        const bool even = (p & 1u) == 0u;
+        lps_dev::launch_radix_pass(s, even ? c->keysA : c->keysB, even ? c->valsA : c->valsB,
+                                   even ? c->keysB : c->keysA, even ? c->valsB : c->valsA, n, 8u * p, c->grid,
+                                   c->hist);
+    }
+    {
+        Timed t(c, "reorder");
+        lps_dev::ReorderArgs r;
+        const int a = c->cur, b = 1 - c->cur;
+        r.keysA = c->keysA; r.valsA = c->valsA; r.keysB = c->keysB; r.valsB = c->valsB;
+        r.n = n;
+        r.g = c->grid;
+        r.pos_in = c->pos[a]; r.vel_in = c->vel[a]; r.vh_in = c->vh[a]; r.first_in = c->first[a];
+        r.id_in = c->id[a]; r.pprev_in = c->pprev[a];
+        r.pos_out = c->pos[b]; r.vel_out = c->vel[b]; r.vh_out = c->vh[b]; r.first_out = c->first[b];
+        r.id_out = c->id[b]; r.pprev_out = c->pprev[b];
+        r.run_end = c->run_end;
+        r.skeys = c->skeys;
+        r.perm = c->perm;
+        lps_dev::launch_reorder(s, r);
+        c->cur = b;
+    }
+    {
+        Timed t(c, "cell_table");
+        lps_dev::launch_cell_table(s, c->run_end, c->grid, c->block_max, c->cell_lb, c->cell_cap);
+    }
+}
+
+struct StepResult {
+    int32_t iterations = 0;
+    uint32_t err_bits = 0;
+};
+
+StepResult enqueue_step(lps_ctx* c, float dt) {
+    StepResult res;
+    const hipStream_t s = c->stream;
+    const bool pci = c->params.method == LPS_METHOD_PCISPH;
+    const float delta = pci ? lps_compute_delta(&c->params, &c->der, dt) : 0.f;
+    enqueue_grid(c);
+    const SimConst k = make_const(c, dt, delta);
+    lps_dev::PhysArgs a = phys_args(c, k);
+    {
+        Timed t(c, "density");
+        lps_dev::launch_density(s, a.pos, c->grid, c->cell_lb, k, c->tables, c->posrho, c->press, c->q);
+    }
+    if (!pci) {
+        {
+            Timed t(c, "forces");
+            lps_dev::launch_forces(s, a, nullptr);
+        }
+        {
+            Timed t(c, "pressure_sph");
+            lps_dev::launch_pressure_sph(s, a, c->q);
+        }
+        {
+            Timed t(c, "integrate");
+            lps_dev::launch_integrate(s, a, nullptr, c->pprev[c->cur], c->press, c->parity ^ 1u);
+        }
+        c->parity ^= 1u;
+        c->last_fp_valid = 0;
+        return res;
+    }
+    {
+        Timed t(c, "forces");
+        lps_dev::launch_forces(s, a, c->xpred[0]);
+    }
+    const uint32_t kmin = (uint32_t)std::max(1, c->params.min_iterations);
+    const uint32_t kmax = (uint32_t)std::max(c->params.max_iterations, (int32_t)kmin);
+    auto enqueue_iter = [&](uint32_t kk) {
+        const uint32_t slot = std::min(kk - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
+        {
+            Timed t(c, "pci_density");
+            lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk == 1u, slot);
+        }
+        {
+            Timed t(c, "pci_pressure");
+            lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp,
+                                         c->xpred[kk & 1u]);
+        }
+        lps_dev::launch_pci_check(s, c->grid, kk, slot, kmin, c->params.rest_density,
+                                  c->params.max_density_error_ratio, c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
+        (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
+    };
+    for (uint32_t kk = 1; kk <= kmin; kk++) enqueue_iter(kk);
+    uint32_t k_enq = kmin, k_known = 0;
+    while (true) {
+        if (k_enq < kmax) enqueue_iter(++k_enq);
+        const uint32_t kw = std::max(k_known + 1u, kmin);
+        (void)hipEventSynchronize(c->iter_ev[kw % LPS_MAX_ITER_SLOTS]);
+        volatile uint32_t* f = c->flags + 4 * (kw % LPS_MAX_ITER_SLOTS);
+        const uint32_t state = f[0];
+        res.iterations = (int32_t)f[1];
+        res.err_bits = f[2];
+        if (state != 0u) break;
+        k_known = kw;
+        if (kw >= kmax) {
+            res.iterations = (int32_t)kmax;
+            break;
+        }
+    }
+    {
+        Timed t(c, "integrate");
+        lps_dev::launch_integrate(s, a, c->fp, c->pprev[c->cur], c->press, c->parity ^ 1u);
+    }
+    c->parity ^= 1u;
+    c->last_fp_valid = 1;
+    return res;
+}
+
+}  // namespace
+
+// ================================================================= C ABI ==
+extern "C" {
+
+const char* lps_status_string(int status) {
+    switch (status) {
+        case LPS_OK: return "LPS_OK";
+        case LPS_ERR_INVALID: return "LPS_ERR_INVALID";
+        case LPS_ERR_NO_DEVICE: return "LPS_ERR_NO_DEVICE";
+        case LPS_ERR_OOM: return "LPS_ERR_OOM";
+        case LPS_ERR_NONFINITE: return "LPS_ERR_NONFINITE";
+        case LPS_ERR_GRID_OVERFLOW: return "LPS_ERR_GRID_OVERFLOW";
+        case LPS_ERR_HIP: return "LPS_ERR_HIP";
+        case LPS_ERR_STATE: return "LPS_ERR_STATE";
+        default: return "LPS_ERR_UNKNOWN";
+    }
+}
+
+int lps_device_info(char* name, size_t cap, int32_t* cu_count) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return LPS_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return LPS_ERR_NO_DEVICE;
+    if (name && cap) {
+        std::snprintf(name, cap, "%s", prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    return LPS_OK;
+}
+
+int lps_create(const lps_params* params, lps_ctx** out) {
+    if (!out) return LPS_ERR_INVALID;
+    *out = nullptr;
+    int st = check_params(params);
+    if (st) return st;
+    char arch[256] = {0};
+    if (lps_device_info(arch, sizeof(arch), nullptr) != LPS_OK) return LPS_ERR_NO_DEVICE;
+    if (std::strncmp(arch, "gfx950", 6) != 0) return LPS_ERR_NO_DEVICE;
+    lps_ctx* c = new (std::nothrow) lps_ctx();
+    if (!c) return LPS_ERR_OOM;
+    c->params = *params;
+    lps_compute_derived(&c->params, &c->der);
+    (void)hipGetDevice(&c->device);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LPS_ERR_HIP;
+    }
+    bool ok = hipMalloc((void**)&c->grid, sizeof(DevGrid)) == hipSuccess &&
+              hipHostMalloc((void**)&c->flags, sizeof(uint32_t) * 4 * LPS_MAX_ITER_SLOTS,
+                            hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
+              hipHostMalloc((void**)&c->h_grid, sizeof(DevGrid), hipHostMallocDefault) == hipSuccess &&
+              hipMemsetAsync(c->grid, 0, sizeof(DevGrid), c->stream) == hipSuccess &&
+              upload_tables(c) == LPS_OK;
+    for (int k = 0; ok && k < LPS_MAX_ITER_SLOTS; k++)
+        ok = hipEventCreateWithFlags(&c->iter_ev[k], hipEventDisableTiming) == hipSuccess;
+    if (ok) {
+        ok = hipMalloc((void**)&c->boxes, sizeof(lps_box) * LPS_MAX_COLLIDERS) == hipSuccess &&
+             hipMalloc((void**)&c->spheres, sizeof(lps_sphere) * LPS_MAX_COLLIDERS) == hipSuccess &&
+             hipMalloc((void**)&c->planes, sizeof(lps_plane) * LPS_MAX_COLLIDERS) == hipSuccess;
+    }
+    if (!ok) {
+        lps_destroy(c);
+        return LPS_ERR_OOM;
+    }
+    std::memset(c->flags, 0, sizeof(uint32_t) * 4 * LPS_MAX_ITER_SLOTS);
+    *out = c;
+    return LPS_OK;
+}
+
+int lps_destroy(lps_ctx* c) {
+    if (!c) return LPS_ERR_INVALID;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++)
+        if (c->iter_ev[k]) (void)hipEventDestroy(c->iter_ev[k]);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return LPS_OK;
+}
+
+int lps_set_params(lps_ctx* c, const lps_params* p) {
+    if (!c) return LPS_ERR_INVALID;
+    int st = check_params(p);
+    if (st) return st;
+    c->params = *p;
+    lps_compute_derived(&c->params, &c->der);
+    return upload_tables(c);
+}
+
+int lps_get_derived(lps_ctx* c, float* h, float* div, float* mass, float* tables) {
+    if (!c) return LPS_ERR_INVALID;
+    if (h) *h = c->der.h;
+    if (div) *div = c->der.div;
+    if (mass) *mass = c->der.mass;
+    if (tables) std::memcpy(tables, c->der.tables, sizeof(c->der.tables));
+    return LPS_OK;
+}
+
+int lps_get_delta(lps_ctx* c, float dt, float* delta) {
+    if (!c || !delta) return LPS_ERR_INVALID;
+    *delta = lps_compute_delta(&c->params, &c->der, dt);
+    return LPS_OK;
+}
+
+int lps_set_colliders(lps_ctx* c, const lps_box* boxes, uint32_t nbox, const lps_sphere* spheres, uint32_t nsphere,
+                      const lps_plane* planes, uint32_t nplane) {
+    if (!c || nbox > LPS_MAX_COLLIDERS || nsphere > LPS_MAX_COLLIDERS || nplane > LPS_MAX_COLLIDERS)
+        return LPS_ERR_INVALID;
+    if ((nbox && !boxes) || (nsphere && !spheres) || (nplane && !planes)) return LPS_ERR_INVALID;
+    if (nbox) HIPC(hipMemcpyAsync(c->boxes, boxes, sizeof(lps_box) * nbox, hipMemcpyHostToDevice, c->stream));
+    if (nsphere)
+        HIPC(hipMemcpyAsync(c->spheres, spheres, sizeof(lps_sphere) * nsphere, hipMemcpyHostToDevice, c->stream));
+    if (nplane) HIPC(hipMemcpyAsync(c->planes, planes, sizeof(lps_plane) * nplane, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    c->nbox = nbox;
+    c->nsphere = nsphere;
+    c->nplane = nplane;
+    c->h_boxes.assign(boxes, boxes + nbox);
+    return LPS_OK;
+}
+
+int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* vhalf, const uint8_t* first,
+                      uint32_t n) {
+    if (!c || !x || n == 0) return LPS_ERR_INVALID;
+    int st = ensure_particles(c, n);
+    if (st) return st;
+    if ((st = ensure_cells(c, cell_bound(c, x, n)))) return st;
+    std::vector<float4> buf(n);
+    auto up4 = [&](float4* dst, const float* src) -> int {
+        for (uint32_t i = 0; i < n; i++)
+            buf[i] = src ? make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+        HIPC(hipMemcpy(dst, buf.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        return LPS_OK;
+    };
+    c->cur = 0;
+    if ((st = up4(c->pos[0], x))) return st;
+    if ((st = up4(c->vel[0], v))) return st;
+    if ((st = up4(c->vh[0], vhalf))) return st;
+    std::vector<uint8_t> f(n, 1);
+    if (first)
+        for (uint32_t i = 0; i < n; i++) f[i] = first[i] ? 1 : 0;
+    HIPC(hipMemcpy(c->first[0], f.data(), n, hipMemcpyHostToDevice));
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; i++) ids[i] = i;
+    HIPC(hipMemcpy(c->id[0], ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemset(c->pprev[0], 0, sizeof(float) * n));
+    c->n = n;
+    c->parity = 0;
+    c->have_step = false;
+    lps_dev::launch_bbox_reset(c->stream, c->grid, 0);
+    lps_dev::launch_bbox(c->stream, c->pos[0], n, c->grid, 0);
+    HIPC(hipStreamSynchronize(c->stream));
+    return LPS_OK;
+}
+
+int lps_get_count(lps_ctx* c, uint32_t* n) {
+    if (!c || !n) return LPS_ERR_INVALID;
+    *n = c->n;
+    return LPS_OK;
+}
+
+int lps_get_particles(lps_ctx* c, float* x, float* v, float* vhalf, uint8_t* first, uint32_t* id, uint32_t n) {
+    if (!c || n != c->n || n == 0) return LPS_ERR_INVALID;
+    HIPC(hipStreamSynchronize(c->stream));
+    std::vector<float4> buf(n);
+    auto down4 = [&](float* dst, const float4* src) -> int {
+        if (!dst) return LPS_OK;
+        HIPC(hipMemcpy(buf.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) {
+            dst[3 * i] = buf[i].x;
+            dst[3 * i + 1] = buf[i].y;
+            dst[3 * i + 2] = buf[i].z;
+        }
+        return LPS_OK;
+    };
+    int st;
+    if ((st = down4(x, c->pos[c->cur]))) return st;
+    if ((st = down4(v, c->vel[c->cur]))) return st;
+    if ((st = down4(vhalf, c->vh[c->cur]))) return st;
+    if (first) HIPC(hipMemcpy(first, c->first[c->cur], n, hipMemcpyDeviceToHost));
+    if (id) HIPC(hipMemcpy(id, c->id[c->cur], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    return LPS_OK;
+}
+
+int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
+    if (!c) return LPS_ERR_INVALID;
+    if (c->n == 0) return LPS_ERR_STATE;
+    if (!(dt > 0.f)) return LPS_ERR_INVALID;
+    const bool pci = c->params.method == LPS_METHOD_PCISPH;
+    // A failed step leaves the state as it was (its kernels no-op and the
+    // reorder copies the identity), so after a grid overflow the cell table is
+    // grown and the remaining steps are issued again.
+    HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    const uint32_t base = c->h_grid->steps_ok;
+    uint32_t completed = 0;
+    int retries = 0;
+    while (completed < nsteps) {
+        const uint32_t batch = pci ? 1u : nsteps - completed;
+        for (uint32_t b = 0; b < batch; b++) {
+            StepResult r = enqueue_step(c, dt);
+            if (stats) {
+                lps_step_stats& o = stats[completed + b];
+                o.iterations = r.iterations;
+                o.max_density_error_ratio = __builtin_bit_cast(float, r.err_bits) / c->params.rest_density;
+                o.warm_start_used = 0;
+                o.status = LPS_OK;
+            }
+        }
+        HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        completed = c->h_grid->steps_ok - base;
+        c->steps = c->steps + batch;
+        c->have_step = true;
+        const uint32_t status = c->h_grid->status;
+        if (status == LPS_ERR_GRID_OVERFLOW) {
+            const uint32_t need = c->h_grid->cells_needed;
+            if (need >= 0x7FFFFFF0u || ++retries > 8) return LPS_ERR_GRID_OVERFLOW;
+            int st = ensure_cells(c, need + need / 4 + 64);
+            if (st) return st;
+            continue;
+        }
+        if (status != 0u) {
+            if (stats && completed < nsteps) stats[completed].status = (int32_t)status;
+            return (int)status;
+        }
+    }
+    if (c->prof) prof_collect(c);
+    return LPS_OK;
+}
+
+int lps_get_grid(lps_ctx* c, lps_grid_info* info, uint32_t* keys, uint32_t* perm, uint32_t* cell_start,
+                 uint32_t* cell_end, uint32_t cap) {
+    if (!c || !c->have_step) return LPS_ERR_STATE;
+    HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    const DevGrid& G = *c->h_grid;
+    if (info) {
+        for (int a = 0; a < 3; a++) {
+            info->bbox_min[a] = G.bbox_min[a];
+            info->bbox_max[a] = G.bbox_max[a];
+            info->grid_offset[a] = G.offset[a];
+            info->grid_size[a] = G.gsize[a];
+        }
+        info->grid_spacing = G.spacing;
+        info->cell_count = G.cells;
+    }
+    const uint32_t n = c->n;
+    if (keys) HIPC(hipMemcpy(keys, c->skeys, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (perm) HIPC(hipMemcpy(perm, c->perm, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (cell_start || cell_end) {
+        const uint32_t m = std::min(cap, G.cells);
+        std::vector<uint32_t> lb((size_t)m + 1);
+        HIPC(hipMemcpy(lb.data(), c->cell_lb, sizeof(uint32_t) * ((size_t)m + 1), hipMemcpyDeviceToHost));
+        for (uint32_t k = 0; k < m; k++) {
+            if (cell_start) cell_start[k] = lb[k];
+            if (cell_end) cell_end[k] = lb[k + 1];
+        }
+    }
+    return LPS_OK;
+}
+
+int lps_get_fields(lps_ctx* c, float* rho, float* pressure, float* force, float* rho_pred) {
+    if (!c || !c->have_step) return LPS_ERR_STATE;
+    HIPC(hipStreamSynchronize(c->stream));
+    const uint32_t n = c->n;
+    std::vector<float4> a(n), b;
+    if (rho || force) HIPC(hipMemcpy(a.data(), c->posrho, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    if (rho)
+        for (uint32_t i = 0; i < n; i++) rho[i] = a[i].w;
+    if (pressure) HIPC(hipMemcpy(pressure, c->press, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (force) {
+        HIPC(hipMemcpy(a.data(), c->force, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        if (c->last_fp_valid) {
+            b.resize(n);
+            HIPC(hipMemcpy(b.data(), c->fp, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        }
+        for (uint32_t i = 0; i < n; i++) {
+            float fx = a[i].x, fy = a[i].y, fz = a[i].z;
+            if (c->last_fp_valid) {
+                fx += b[i].x;
+                fy += b[i].y;
+                fz += b[i].z;
+            }
+            force[3 * i] = fx;
+            force[3 * i + 1] = fy;
+            force[3 * i + 2] = fz;
+        }
+    }
+    if (rho_pred) HIPC(hipMemcpy(rho_pred, c->rho_pred, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return LPS_OK;
+}
+
+int lps_mlp_set_weights(lps_ctx* c, const float* params, const float* feature_scale, float target_scale) {
+    if (!c || !params || !feature_scale || !(target_scale > 0.f)) return LPS_ERR_INVALID;
+    c->mlp_w.assign(params, params + LPS_MLP_PARAMS);
+    c->mlp_fscale.assign(feature_scale, feature_scale + 2 * LPS_MLP_IN);
+    c->mlp_tscale = target_scale;
+    c->mlp_loaded = true;
+    return LPS_OK;
+}
+
+int lps_mlp_get_weights(lps_ctx* c, float* params, float* feature_scale, float* target_scale) {
+    if (!c) return LPS_ERR_INVALID;
+    if (!c->mlp_loaded) return LPS_ERR_STATE;
+    if (params) std::memcpy(params, c->mlp_w.data(), sizeof(float) * LPS_MLP_PARAMS);
+    if (feature_scale) std::memcpy(feature_scale, c->mlp_fscale.data(), sizeof(float) * 2 * LPS_MLP_IN);
+    if (target_scale) *target_scale = c->mlp_tscale;
+    return LPS_OK;
+}
+
+int lps_dump_pairs(lps_ctx* c, float* features, float* target, uint32_t n) {
+    if (!c || !features || !target) return LPS_ERR_INVALID;
+    if (n != c->n) return LPS_ERR_INVALID;
+    return LPS_ERR_STATE;
+}
+
+int lps_profile_enable(lps_ctx* c, int32_t on) {
+    if (!c) return LPS_ERR_INVALID;
+    if (!on) prof_collect(c);
+    c->prof = on != 0;
+    return LPS_OK;
+}
+
+int lps_profile_query(lps_ctx* c, const char* kernel, uint32_t* launches, double* total_ms) {
+    if (!c || !kernel) return LPS_ERR_INVALID;
+    prof_collect(c);
+    auto it = c->prof_acc.find(kernel);
+    if (launches) *launches = it == c->prof_acc.end() ? 0u : it->second.first;
+    if (total_ms) *total_ms = it == c->prof_acc.end() ? 0.0 : it->second.second;
+    return LPS_OK;
+}
+
+int lps_profile_reset(lps_ctx* c) {
+    if (!c) return LPS_ERR_INVALID;
+    prof_collect(c);
+    c->prof_acc.clear();
+    return LPS_OK;
+}
+
+int lps_get_stream(lps_ctx* c, void** stream) {
+    if (!c || !stream) return LPS_ERR_INVALID;
+    *stream = (void*)c->stream;
+    return LPS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,88 @@
+// lps_common.h -- types shared by the host orchestration and the gfx950 kernels.
+//
+// Floating-point rule for every kernel: no FMA contraction (the library is
+// compiled with -ffp-contract=off), IEEE division and square root (HIP's
+// default correctly rounded fp32 div/sqrt), and the evaluation orders of the
+// reference's OpenCL kernels (LiquidPhysics/cl_kernels/SPHKernels.cl), so the
+// device results equal an IEEE single-precision evaluation of those orders.
+#pragma once
+#include <stdint.h>
+
+#include "lps.h"
+
+#define LPS_BLOCK 256           // threads per block for particle kernels (4 wave64)
+#define LPS_RADIX_ITEMS 8       // keys per thread per radix tile
+#define LPS_RADIX_TILE (LPS_BLOCK * LPS_RADIX_ITEMS)
+#define LPS_SCAN_ITEMS 16       // cells per thread in the cell-table scan
+#define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)
+#define LPS_MAX_ITER_SLOTS 64   // per-iteration error slots (iterations beyond reuse the last)
+#define LPS_MAX_COLLIDERS 64
+
+// Device-resident grid state of the current step.  Written by the bbox
+// reduction (integrate of the previous step, or the bbox kernel) and by the
+// grid set-up kernel; read by every later kernel of the step.
+struct DevGrid {
+    // two bbox accumulators, alternating by step parity; orderable-uint encoded floats
+    uint32_t bmin[2][3];
+    uint32_t bmax[2][3];
+    uint32_t nonfinite[2];
+    // grid of the current step (SPHSolver.cpp:1031-1040)
+    float bbox_min[3];
+    float bbox_max[3];
+    float offset[3];
+    float spacing;
+    uint32_t gsize[3];
+    uint32_t cells;
+    uint32_t key_bits;       // bits of (cells - 1): radix passes beyond are skipped
+    uint32_t passes_done;    // radix passes executed this step
+    uint32_t status;         // lps_status of the step; kernels no-op when non-zero
+    uint32_t cells_needed;   // cell count requested when status == GRID_OVERFLOW
+    uint32_t err_bits[LPS_MAX_ITER_SLOTS];  // max |e_i| (float bits) per PCISPH iteration
+    uint32_t done;           // PCISPH loop converged (or stopped): later corrections no-op
+    uint32_t iterations;     // corrections run when done was set
+    uint32_t steps_ok;       // steps whose grid set-up succeeded (monotone counter)
+};
+
+// Per-step constants passed by value (the role of the 128-byte
+// ParallelSPHParameters block, ParallelSPHStructs.h:17-40).
+struct SimConst {
+    float gravity[3];
+    float particle_radius;
+    float h;            // kernel radius
+    float div;          // table step h / 1000
+    float s_max;        // largest float s with sqrtf(s) < h: d < h  <=>  d^2 <= s_max
+    float mass;
+    float rest_density;
+    float stiffness;
+    float npf;          // negative pressure factor
+    float viscosity;
+    float surface_tension;
+    float st_threshold;
+    float restitution;
+    float friction;
+    float dt;
+    float delta;        // PCISPH delta (PCISPHSolver.cpp:286-328)
+    uint32_t n;
+    uint32_t nbox, nsphere, nplane;
+    uint32_t cell_cap;  // allocated cell-table entries (excluding the sentinel)
+};
+
+// orderable encoding of floats for atomic min/max (monotone for non-NaN)
+static inline __host__ __device__ uint32_t f2o(float f) {
+    union { float f; uint32_t u; } c;
+    c.f = f;
+    return (c.u & 0x80000000u) ? ~c.u : (c.u | 0x80000000u);
+}
+static inline __host__ __device__ float o2f(uint32_t u) {
+    union { float f; uint32_t u; } c;
+    c.u = (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+    return c.f;
+}
+
+// Host-side derived quantities (lps_host.cpp).
+struct Derived {
+    float h, div, mass, s_max;
+    float tables[5][LPS_TABLE_SIZE];  // W, W', W'' (poly6), spiky', viscosity''
+};
+void lps_compute_derived(const lps_params* p, Derived* d);
+float lps_compute_delta(const lps_params* p, const Derived* d, float dt);

@@ -1,0 +1,1004 @@
+// lps_kernels.hip -- hand-written gfx950 kernels of the SPH/PCISPH step.
+//
+// One launch per phase of PhysicSolver::update (LiquidPhysics/src/PhysicSolver.cpp:12-21):
+//   grid      k_bbox, k_grid_setup, k_keys, k_radix_{hist,scan,scatter} x P,
+//             k_reorder (+ run-end marks), k_cell_{reduce,scan}
+//   density   k_density
+//   forces    k_forces (+ first PCISPH prediction)
+//   pressure  SPH: k_pressure_sph;  PCISPH: {k_pci_density, k_pci_pressure (+ next
+//             prediction), k_pci_check} per correction
+//   integrate k_integrate (+ collisions, + bounding box of the next step)
+// Particle kernels run 256 threads (4 wave64) per block, one particle per thread,
+// with a block index remapped so each of the 8 XCDs works on a contiguous range
+// of the cell-sorted particles and keeps its neighbours in its own L2.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "lps_common.h"
+#include "lps_kernels.h"
+
+#define LPS_XCDS 8u
+
+namespace {
+
+// ------------------------------------------------------------ utilities --
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Logical block of an XCD-contiguous mapping: physical block b runs on XCD b % 8;
+// XCD k gets logical blocks [k*per, (k+1)*per).  Returns UINT_MAX for padding blocks.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t nlogical) {
+    const uint32_t per = (nlogical + LPS_XCDS - 1) / LPS_XCDS;
+    const uint32_t b = blockIdx.x;
+    const uint32_t lb = (b % LPS_XCDS) * per + b / LPS_XCDS;
+    return lb < nlogical ? lb : UINT_MAX;
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
+    return v;
+}
+
+// 64-lane mask of the lanes holding the same 8-bit digit (8 ballots).
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
+    uint64_t peers = active;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        const int bit = (d >> b) & 1u;
+        const uint64_t m = __ballot(bit);
+        peers &= bit ? m : ~m;
+    }
+    return peers;
+}
+
+__device__ __forceinline__ void load_table(float* dst, const float* src) {
+    for (uint32_t k = threadIdx.x; k < LPS_TABLE_SIZE; k += blockDim.x) dst[k] = src[k];
+}
+
+// ---------------------------------------------------- grid traversal --
+struct GridView {
+    float off0, off1, off2, gs;
+    int gx, gy, gz;
+    uint32_t gxy;
+};
+
+__device__ __forceinline__ GridView grid_view(const DevGrid* g) {
+    GridView v;
+    v.off0 = g->offset[0];
+    v.off1 = g->offset[1];
+    v.off2 = g->offset[2];
+    v.gs = g->spacing;
+    v.gx = (int)g->gsize[0];
+    v.gy = (int)g->gsize[1];
+    v.gz = (int)g->gsize[2];
+    v.gxy = g->gsize[0] * g->gsize[1];
+    return v;
+}
+
+// The 27-cell stencil of the OpenCL kernels (e.g. SPHKernels.cl:219-244):
+// cells z-outer, y, x-inner around the cell of `anchor`, each cell's sorted
+// range ascending.  Cells x-1, x, x+1 of one row are consecutive keys, so a
+// row is the single range [LB[k(x-1)], LB[k(x+1)+1]) with the same visiting order.
+template <class F>
+__device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t* __restrict__ lb, float ax,
+                                              float ay, float az, F&& fn) {
+    const int xg = (int)truncf((ax + G.off0) / G.gs);
+    const int yg = (int)truncf((ay + G.off1) / G.gs);
+    const int zg = (int)truncf((az + G.off2) / G.gs);
+    const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
+    for (int z = zg - 1; z <= zg + 1; z++) {
+        if (z < 0 || z >= G.gz) continue;
+        for (int y = yg - 1; y <= yg + 1; y++) {
+            if (y < 0 || y >= G.gy) continue;
+            const uint32_t base = (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+            const uint32_t b = lb[base + (uint32_t)x0];
+            const uint32_t e = lb[base + (uint32_t)x1 + 1u];
+            for (uint32_t j = b; j < e; j++) fn(j);
+        }
+    }
+}
+
+// ---------------------------------------------------------- colliders --
+// resolveCollision (SPHKernels.cl:640-659)
+__device__ __forceinline__ void resolve(float3& pos, float3& vel, float e, float mu, float3 n, float3 pt) {
+    const float sv = vel.x * n.x + vel.y * n.y + vel.z * n.z;
+    if (sv < 0.f) {
+        float3 vn = make_float3(n.x * sv, n.y * sv, n.z * sv);
+        float3 vt = make_float3(vel.x - vn.x, vel.y - vn.y, vel.z - vn.z);
+        vn.x *= -e; vn.y *= -e; vn.z *= -e;
+        vt.x *= mu; vt.y *= mu; vt.z *= mu;
+        vel = make_float3(vn.x + vt.x, vn.y + vt.y, vn.z + vt.z);
+    }
+    pos = pt;
+}
+
+// handleCollisionWithBox (SPHKernels.cl:488-598); the boundary normal is
+// normalised as v * (1/|v|).
+__device__ __forceinline__ void collide_box(float3& pos, float3& vel, float r, float e, float mu,
+                                            const lps_box& b) {
+    const float cx = b.center[0], cy = b.center[1], cz = b.center[2];
+    const float hx = b.half[0], hy = b.half[1], hz = b.half[2];
+    const float dx = fabsf(pos.x - cx), dy = fabsf(pos.y - cy), dz = fabsf(pos.z - cz);
+    bool hit = false;
+    float3 n = make_float3(0.f, 0.f, 0.f);
+    float3 pt = make_float3(0.f, 0.f, 0.f);
+    if (!b.boundary) {
+        const float px = (hx - dx) + r, py = (hy - dy) + r, pz = (hz - dz) + r;
+        if (px > 0.f && py > 0.f && pz > 0.f) {
+            hit = true;
+            pt = pos;
+            if (px <= py && px <= pz) {
+                n.x = (pos.x - cx >= 0.f) ? 1.f : -1.f;
+                pt.x = cx + n.x * (hx + r);
+            } else if (py <= px && py <= pz) {
+                n.y = (pos.y - cy >= 0.f) ? 1.f : -1.f;
+                pt.y = cy + n.y * (hy + r);
+            } else if (pz <= py && pz <= px) {
+                n.z = (pos.z - cz >= 0.f) ? 1.f : -1.f;
+                pt.z = cz + n.z * (hz + r);
+            }
+        }
+    } else {
+        const float px = (dx - hx) + r, py = (dy - hy) + r, pz = (dz - hz) + r;
+        if (px > 0.f || py > 0.f || pz > 0.f) {
+            hit = true;
+            pt = pos;
+            if (px > 0.f) {
+                n.x = (pos.x - cx <= 0.f) ? 1.f : -1.f;
+                pt.x = cx - n.x * (hx - r);
+            }
+            if (py > 0.f) {
+                n.y = (pos.y - cy <= 0.f) ? 1.f : -1.f;
+                pt.y = cy - n.y * (hy - r);
+            }
+            if (pz > 0.f) {
+                n.z = (pos.z - cz <= 0.f) ? 1.f : -1.f;
+                pt.z = cz - n.z * (hz - r);
+            }
+            const float m = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+            if (m > 0.f) {
+                const float inv = 1.f / m;
+                n.x *= inv; n.y *= inv; n.z *= inv;
+            }
+        }
+    }
+    if (hit) resolve(pos, vel, e, mu, n, pt);
+}
+
+// handleCollisionWithSphere (SPHKernels.cl:600-638): normal normalised first.
+__device__ __forceinline__ void collide_sphere(float3& pos, float3& vel, float r, float e, float mu,
+                                               const lps_sphere& s) {
+    const float3 c = make_float3(s.center[0], s.center[1], s.center[2]);
+    const float R = s.radius;
+    const float ddx = c.x - pos.x, ddy = c.y - pos.y, ddz = c.z - pos.z;
+    const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+    if (!s.boundary) {
+        if ((R + r) - d > 0.f) {
+            const float3 n = make_float3((pos.x - c.x) / d, (pos.y - c.y) / d, (pos.z - c.z) / d);
+            const float k = R + r;
+            resolve(pos, vel, e, mu, n, make_float3(c.x + n.x * k, c.y + n.y * k, c.z + n.z * k));
+        }
+    } else {
+        if (d - (R - r) > 0.f) {
+            const float3 n = make_float3((c.x - pos.x) / d, (c.y - pos.y) / d, (c.z - pos.z) / d);
+            const float k = R - r;
+            resolve(pos, vel, e, mu, n, make_float3(c.x - n.x * k, c.y - n.y * k, c.z - n.z * k));
+        }
+    }
+}
+
+// Plane (no reference counterpart): fluid side n.x >= offset; penetration
+// r - (n.x - offset); contact point moved along n; resolveCollision response.
+__device__ __forceinline__ void collide_plane(float3& pos, float3& vel, float r, float e, float mu,
+                                              const lps_plane& pl) {
+    const float3 n = make_float3(pl.normal[0], pl.normal[1], pl.normal[2]);
+    const float s = (n.x * pos.x + n.y * pos.y + n.z * pos.z) - pl.offset;
+    const float pen = r - s;
+    if (pen > 0.f)
+        resolve(pos, vel, e, mu, n, make_float3(pos.x + n.x * pen, pos.y + n.y * pen, pos.z + n.z * pen));
+}
+
+// handleCollisions order: boxes, spheres (SPHKernels.cl:472-480), then planes.
+__device__ __forceinline__ void collide_all(float3& pos, float3& vel, const SimConst& c, const lps_box* boxes,
+                                            const lps_sphere* spheres, const lps_plane* planes) {
+    for (uint32_t j = 0; j < c.nbox; j++) collide_box(pos, vel, c.particle_radius, c.restitution, c.friction, boxes[j]);
+    for (uint32_t j = 0; j < c.nsphere; j++)
+        collide_sphere(pos, vel, c.particle_radius, c.restitution, c.friction, spheres[j]);
+    for (uint32_t j = 0; j < c.nplane; j++)
+        collide_plane(pos, vel, c.particle_radius, c.restitution, c.friction, planes[j]);
+}
+
+// pciIntegrate + pciHandleCollisions (SPHKernels.cl:676-735): predicted position
+// from acceleration (f + fp) / rho; the first-step back-step reads v.
+__device__ __forceinline__ float4 predict(float4 x, float4 v, float4 vh, bool first, float fx, float fy, float fz,
+                                          float rho, const SimConst& c, const lps_box* boxes,
+                                          const lps_sphere* spheres, const lps_plane* planes) {
+    const float ax = fx / rho, ay = fy / rho, az = fz / rho;
+    float3 hv = make_float3(vh.x, vh.y, vh.z);
+    if (first) {
+        hv.x = v.x - ax * c.dt / 2.f;
+        hv.y = v.y - ay * c.dt / 2.f;
+        hv.z = v.z - az * c.dt / 2.f;
+    }
+    hv.x += ax * c.dt; hv.y += ay * c.dt; hv.z += az * c.dt;
+    float3 p = make_float3(x.x + hv.x * c.dt, x.y + hv.y * c.dt, x.z + hv.z * c.dt);
+    collide_all(p, hv, c, boxes, spheres, planes);
+    return make_float4(p.x, p.y, p.z, 0.f);
+}
+
+__device__ __forceinline__ bool step_dead(const DevGrid* g) { return g->status != 0u; }
+
+}  // namespace
+
+// ================================================================ grid ====
+// Bounding box of the positions (SPHSolver.cpp:1011-1029): min starts at
+// FLT_MAX and max at FLT_MIN as in the reference; non-finite coordinates flag
+// the step.  Accumulates into slot `slot` of the two alternating accumulators.
+__device__ __forceinline__ void bbox_accumulate(DevGrid* g, uint32_t slot, bool valid, float x, float y, float z) {
+    __shared__ uint32_t s_red[7][LPS_BLOCK / 64];
+    uint32_t mn[3], mx[3], nf = 0;
+    const float v[3] = {x, y, z};
+    for (int a = 0; a < 3; a++) {
+        const bool fin = valid && isfinite(v[a]);
+        if (valid && !isfinite(v[a])) nf = 1;
+        mn[a] = fin ? f2o(v[a]) : 0xFFFFFFFFu;
+        mx[a] = fin ? f2o(v[a]) : 0u;
+        mn[a] = wave_min_u32(mn[a]);
+        mx[a] = wave_max_u32(mx[a]);
+    }
+    nf = wave_max_u32(nf);
+    const uint32_t w = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        for (int a = 0; a < 3; a++) {
+            s_red[a][w] = mn[a];
+            s_red[3 + a][w] = mx[a];
+        }
+        s_red[6][w] = nf;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; a++) {
+            uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+            for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) {
+                lo = min(lo, s_red[a][k]);
+                hi = max(hi, s_red[3 + a][k]);
+            }
+            if (lo != 0xFFFFFFFFu) atomicMin(&g->bmin[slot][a], lo);
+            if (hi != 0u) atomicMax(&g->bmax[slot][a], hi);
+        }
+        uint32_t any = 0;
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) any |= s_red[6][k];
+        if (any) atomicOr(&g->nonfinite[slot], 1u);
+    }
+}
+
+__global__ void k_bbox_reset(DevGrid* g, uint32_t slot) {
+    if (threadIdx.x == 0) {
+        for (int a = 0; a < 3; a++) {
+            g->bmin[slot][a] = f2o(FLT_MAX);
+            g->bmax[slot][a] = f2o(FLT_MIN);
+        }
+        g->nonfinite[slot] = 0u;
+        g->status = 0u;
+    }
+}
+
+__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, uint32_t n, DevGrid* g,
+                                                    uint32_t slot, uint32_t nlogical) {
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < n) p = pos[i];
+    bbox_accumulate(g, slot, i < n, p.x, p.y, p.z);
+}
+
+// Grid parameters (SPHSolver.cpp:1031-1040), one thread.  Also resets the
+// per-step counters and the other bbox accumulator.
+__global__ void k_grid_setup(DevGrid* g, uint32_t slot, float h, uint32_t cell_cap) {
+    if (threadIdx.x != 0) return;
+    uint32_t status = 0;
+    float mn[3], mx[3];
+    for (int a = 0; a < 3; a++) {
+        mn[a] = o2f(g->bmin[slot][a]);
+        mx[a] = o2f(g->bmax[slot][a]);
+        g->bbox_min[a] = mn[a];
+        g->bbox_max[a] = mx[a];
+    }
+    if (g->nonfinite[slot]) status = LPS_ERR_NONFINITE;
+    const float gs = h * 1.f;
+    g->spacing = gs;
+    uint64_t cells = 1;
+    for (int a = 0; a < 3; a++) {
+        g->offset[a] = -mn[a];
+        const float ex = (mx[a] - mn[a]) / gs;
+        if (!(ex < 2.0e9f)) {
+            status = status ? status : LPS_ERR_GRID_OVERFLOW;
+            g->gsize[a] = 1u;
+        } else {
+            g->gsize[a] = (uint32_t)((int)ex + 1);
+        }
+        cells *= g->gsize[a];
+    }
+    g->cells_needed = cells > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cells;
+    if (!status && cells > (uint64_t)cell_cap) status = LPS_ERR_GRID_OVERFLOW;
+    g->cells = status ? 1u : (uint32_t)cells;
+    const uint32_t cm1 = g->cells - 1u;
+    g->key_bits = cm1 ? 32u - (uint32_t)__clz(cm1) : 0u;
+    g->passes_done = 0u;
+    g->status = status;
+    g->done = 0u;
+    g->iterations = 0u;
+    if (!status) g->steps_ok += 1u;
+    for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++) g->err_bits[k] = 0u;
+    const uint32_t o = slot ^ 1u;
+    for (int a = 0; a < 3; a++) {
+        g->bmin[o][a] = f2o(FLT_MAX);
+        g->bmax[o][a] = f2o(FLT_MIN);
+    }
+    g->nonfinite[o] = 0u;
+}
+
+// calcGridIndices (SPHKernels.cl:66-82)
+__global__ __launch_bounds__(LPS_BLOCK) void k_keys(const float4* __restrict__ pos, uint32_t n, const DevGrid* g,
+                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    if (step_dead(g)) return;
+    const uint32_t i = blockIdx.x * LPS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const GridView G = grid_view(g);
+    const float4 p = pos[i];
+    const uint32_t xg = (uint32_t)truncf((p.x + G.off0) / G.gs);
+    const uint32_t yg = (uint32_t)truncf((p.y + G.off1) / G.gs);
+    const uint32_t zg = (uint32_t)truncf((p.z + G.off2) / G.gs);
+    keys[i] = xg + (uint32_t)G.gx * yg + G.gxy * zg;
+    vals[i] = i;
+}
+
+// ---- stable LSD radix sort by key, 8-bit digits, reduce-then-scan ----------
+// A pass is skipped when its digit is above the key width of this step's
+// cell count; skipped passes leave the result in the buffer given by the
+// parity of passes_done.
+__global__ __launch_bounds__(LPS_BLOCK) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n,
+                                                          uint32_t shift, const DevGrid* g,
+                                                          uint32_t* __restrict__ hist, uint32_t ntiles) {
+    if (step_dead(g) || shift >= g->key_bits) return;
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const uint32_t idx = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
+        const bool valid = idx < n;
+        const uint32_t d = valid ? (keys[idx] >> shift) & 0xFFu : 0u;
+        const uint64_t act = __ballot(valid);
+        const uint64_t peers = match_digit(d, act);
+        if (valid && lane_id() == (uint32_t)__ffsll((long long)peers) - 1u) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of the digit-major histogram (the same bucket-major order
+// as the reference's host scan, SPHSolver.cpp:1069-1078), one block.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist, uint32_t total, DevGrid* g,
+                                                     uint32_t shift) {
+    if (step_dead(g) || shift >= g->key_bits) return;
+    __shared__ uint32_t wsum[16];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (total + 1023u) / 1024u;
+    const uint32_t s = min(t * per, total), e = min(s + per, total);
+    uint32_t sum = 0;
+    for (uint32_t k = s; k < e; k++) sum += hist[k];
+    // block exclusive scan of sum
+    uint32_t incl = sum;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+        if (lane_id() >= (uint32_t)off) incl += y;
+    }
+    if (lane_id() == 63) wsum[t >> 6] = incl;
+    __syncthreads();
+    if (t < 64) {
+        uint32_t v = t < 16 ? wsum[t] : 0u;
+        uint32_t inc = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)inc, off, 64);
+            if (lane_id() >= (uint32_t)off) inc += y;
+        }
+        if (t < 16) wsum[t] = inc - v;
+    }
+    __syncthreads();
+    uint32_t off = wsum[t >> 6] + incl - sum;
+    for (uint32_t k = s; k < e; k++) {
+        const uint32_t v = hist[k];
+        hist[k] = off;
+        off += v;
+    }
+    if (t == 0) g->passes_done += 1u;
+}
+
+// Stable scatter of one tile: items are taken in index order (round-major,
+// then wave, then lane); ranks within a wave come from the digit match mask,
+// across waves and rounds from per-digit running offsets in LDS.
+__global__ __launch_bounds__(LPS_BLOCK) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                             const uint32_t* __restrict__ vin,
+                                                             uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                             uint32_t n, uint32_t shift, const DevGrid* g,
+                                                             const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    if (step_dead(g) || shift >= g->key_bits) return;
+    __shared__ uint32_t s_off[256];
+    __shared__ uint32_t s_w[LPS_BLOCK / 64][256];
+    const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
+    const uint32_t tile = blockIdx.x;
+    s_off[t] = hist[t * ntiles + tile];
+    for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) s_w[k][t] = 0u;
+    __syncthreads();
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const uint32_t idx = tile * LPS_RADIX_TILE + r * LPS_BLOCK + t;
+        const bool valid = idx < n;
+        uint32_t key = 0, val = 0;
+        if (valid) {
+            key = kin[idx];
+            val = vin[idx];
+        }
+        const uint32_t d = (key >> shift) & 0xFFu;
+        const uint64_t act = __ballot(valid);
+        const uint64_t peers = match_digit(d, act);
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0u) s_w[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t base = s_off[t];
+            for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) {
+                const uint32_t c = s_w[k][t];
+                s_w[k][t] = base;
+                base += c;
+            }
+            s_off[t] = base;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t dst = s_w[w][d] + rank;
+            kout[dst] = key;
+            vout[dst] = val;
+        }
+        __syncthreads();
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) s_w[k][t] = 0u;
+        __syncthreads();
+    }
+}
+
+// Gather the state into sorted order (the payload moves of permuteParticles,
+// SPHKernels.cl:108-146, done once) and mark the end of every run of equal
+// keys for the cell table.  On a failed step the copy is the identity so the
+// state survives the buffer flip.
+__global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
+    const uint32_t* __restrict__ keysA, const uint32_t* __restrict__ valsA, const uint32_t* __restrict__ keysB,
+    const uint32_t* __restrict__ valsB, uint32_t n, const DevGrid* g, const float4* __restrict__ pos_in,
+    const float4* __restrict__ vel_in, const float4* __restrict__ vh_in, const uint8_t* __restrict__ first_in,
+    const uint32_t* __restrict__ id_in, const float* __restrict__ pprev_in, float4* __restrict__ pos_out,
+    float4* __restrict__ vel_out, float4* __restrict__ vh_out, uint8_t* __restrict__ first_out,
+    uint32_t* __restrict__ id_out, float* __restrict__ pprev_out, uint32_t* __restrict__ run_end,
+    uint32_t* __restrict__ skeys_out, uint32_t* __restrict__ perm_out, uint32_t nlogical) {
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const bool dead = step_dead(g);
+    uint32_t j = i;
+    if (!dead) {
+        const bool b = (g->passes_done & 1u) != 0u;
+        const uint32_t* keys = b ? keysB : keysA;
+        const uint32_t* vals = b ? valsB : valsA;
+        j = vals[i];
+        const uint32_t k = keys[i];
+        skeys_out[i] = k;
+        perm_out[i] = j;
+        if (i == n - 1u || keys[i + 1u] != k) run_end[k] = i + 1u;
+    }
+    pos_out[i] = pos_in[j];
+    vel_out[i] = vel_in[j];
+    vh_out[i] = vh_in[j];
+    first_out[i] = first_in[j];
+    id_out[i] = id_in[j];
+    if (pprev_in) pprev_out[i] = pprev_in[j];
+}
+
+// Cell table = exclusive max-scan of the run ends: LB[c] = #keys < c, the
+// cellList of buildCellList (SPHKernels.cl:148-186), with LB[cells] = n.  The
+// run-end marks are cleared as they are consumed.
+__global__ __launch_bounds__(LPS_BLOCK) void k_cell_reduce(const uint32_t* __restrict__ run_end, const DevGrid* g,
+                                                           uint32_t* __restrict__ block_max) {
+    if (step_dead(g)) return;
+    const uint32_t total = g->cells + 1u;
+    const uint32_t base = blockIdx.x * LPS_SCAN_TILE;
+    if (base >= total) return;
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < LPS_SCAN_ITEMS; k++) {
+        const uint32_t c = base + k * LPS_BLOCK + threadIdx.x;
+        if (c < total) m = max(m, run_end[c]);
+    }
+    m = wave_max_u32(m);
+    __shared__ uint32_t s[LPS_BLOCK / 64];
+    if (lane_id() == 0) s[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) r = max(r, s[k]);
+        block_max[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(LPS_BLOCK) void k_cell_scan(uint32_t* __restrict__ run_end, const DevGrid* g,
+                                                         const uint32_t* __restrict__ block_max,
+                                                         uint32_t* __restrict__ cell_lb) {
+    if (step_dead(g)) return;
+    const uint32_t total = g->cells + 1u;
+    const uint32_t base = blockIdx.x * LPS_SCAN_TILE;
+    if (base >= total) return;
+    __shared__ uint32_t s[LPS_BLOCK / 64];
+    __shared__ uint32_t s_carry;
+    const uint32_t t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    // carry = max of the previous blocks
+    uint32_t c0 = 0;
+    for (uint32_t k = t; k < blockIdx.x; k += LPS_BLOCK) c0 = max(c0, block_max[k]);
+    c0 = wave_max_u32(c0);
+    if (lane == 0) s[w] = c0;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) r = max(r, s[k]);
+        s_carry = r;
+    }
+    __syncthreads();
+    uint32_t carry = s_carry;
+    // thread t owns cells base + t*ITEMS .. +ITEMS (contiguous)
+    const uint32_t c_begin = base + t * LPS_SCAN_ITEMS;
+    uint32_t v[LPS_SCAN_ITEMS];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < LPS_SCAN_ITEMS; k++) {
+        const uint32_t c = c_begin + k;
+        v[k] = (c < total) ? run_end[c] : 0u;
+        local = max(local, v[k]);
+    }
+    // exclusive max-scan of `local` across the block
+    uint32_t incl = local;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+        if (lane >= (uint32_t)off) incl = max(incl, y);
+    }
+    uint32_t excl = (uint32_t)__shfl_up((int)incl, 1, 64);
+    if (lane == 0) excl = 0u;
+    __syncthreads();
+    if (lane == 63) s[w] = incl;
+    __syncthreads();
+    uint32_t wprefix = 0;
+    for (uint32_t k = 0; k < w; k++) wprefix = max(wprefix, s[k]);
+    uint32_t run = max(carry, max(wprefix, excl));
+#pragma unroll
+    for (uint32_t k = 0; k < LPS_SCAN_ITEMS; k++) {
+        const uint32_t c = c_begin + k;
+        if (c < total) {
+            cell_lb[c] = run;
+            if (v[k]) run_end[c] = 0u;
+        }
+        run = max(run, v[k]);
+    }
+}
+
+// ============================================================= physics ====
+// calcDensityPressure (SPHKernels.cl:190-254).  Writes (x, rho) packed for the
+// force pass, the pressure, and q = p / rho^2 (the per-particle term of the
+// SPH pressure force, SPHKernels.cl:383,412).
+__global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict__ pos, const DevGrid* g,
+                                                       const uint32_t* __restrict__ cell_lb, SimConst c,
+                                                       const float* __restrict__ tables,
+                                                       float4* __restrict__ posrho, float* __restrict__ press,
+                                                       float* __restrict__ q, uint32_t nlogical) {
+    if (step_dead(g)) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sW[LPS_TABLE_SIZE];
+    load_table(sW, tables);
+    __syncthreads();
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const GridView G = grid_view(g);
+    const float4 xi = pos[i];
+    float sum = 0.f;
+    for_neighbors(G, cell_lb, xi.x, xi.y, xi.z, [&](uint32_t j) {
+        const float4 xj = pos[j];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float s2 = dx * dx + dy * dy + dz * dz;
+        if (s2 <= c.s_max) {
+            const float d = sqrtf(s2);
+            sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
+        }
+    });
+    const float rho = c.mass * sum;
+    float p = c.stiffness * (rho - c.rest_density);
+    if (p < 0.f) p *= c.npf;
+    posrho[i] = make_float4(xi.x, xi.y, xi.z, rho);
+    press[i] = p;
+    q[i] = p / (rho * rho);
+}
+
+// accumulateNonPressureForces (SPHKernels.cl:256-349); for PCISPH also the
+// first prediction of the correction loop (pciInit + pciIntegrate +
+// pciHandleCollisions with fp = 0, SPHKernels.cl:661-735).
+__global__ __launch_bounds__(LPS_BLOCK) void k_forces(
+    const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
+    const uint8_t* __restrict__ first, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c,
+    const float* __restrict__ tables, const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres,
+    const lps_plane* __restrict__ planes, float4* __restrict__ force, float4* __restrict__ xpred,
+    uint32_t nlogical) {
+    if (step_dead(g)) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE];
+    load_table(sW1, tables + 1 * LPS_TABLE_SIZE);
+    load_table(sW2, tables + 2 * LPS_TABLE_SIZE);
+    load_table(sV2, tables + 4 * LPS_TABLE_SIZE);
+    __syncthreads();
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const GridView G = grid_view(g);
+    const float4 xi = posrho[i];
+    const float4 vi = vel[i];
+    const float ri = xi.w;
+    float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
+    float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
+    for_neighbors(G, cell_lb, xi.x, xi.y, xi.z, [&](uint32_t j) {
+        const float4 xj = posrho[j];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float s2 = dx * dx + dy * dy + dz * dz;
+        if (s2 <= c.s_max) {
+            const float d = sqrtf(s2);
+            const uint32_t idx = min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1));
+            const float rj = xj.w;
+            const float w1 = sW1[idx];
+            snx += (xi.x - xj.x) * w1 / rj;
+            sny += (xi.y - xj.y) * w1 / rj;
+            snz += (xi.z - xj.z) * w1 / rj;
+            lc += sW2[idx] / rj;
+            if (i != j) {
+                const float4 vj = vel[j];
+                const float wv = sV2[idx];
+                vfx += (vj.x - vi.x) * wv / rj;
+                vfy += (vj.y - vi.y) * wv / rj;
+                vfz += (vj.z - vi.z) * wv / rj;
+            }
+        }
+    });
+    snx *= c.mass; sny *= c.mass; snz *= c.mass;
+    const float snl = sqrtf(snx * snx + sny * sny + snz * snz);
+    if (snl > c.st_threshold) {
+        const float k = -c.surface_tension * lc * c.mass;
+        fx += (snx / snl) * k;
+        fy += (sny / snl) * k;
+        fz += (snz / snl) * k;
+    }
+    fx += vfx * c.viscosity * c.mass;
+    fy += vfy * c.viscosity * c.mass;
+    fz += vfz * c.viscosity * c.mass;
+    force[i] = make_float4(fx, fy, fz, 0.f);
+    if (xpred) {
+        const float4 xo = make_float4(xi.x, xi.y, xi.z, 0.f);
+        xpred[i] = predict(xo, vi, vh[i], first[i] != 0, fx + 0.f, fy + 0.f, fz + 0.f, ri, c, boxes, spheres, planes);
+    }
+}
+
+// accumulatePressureForces (SPHKernels.cl:351-422), SPH only.
+__global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __restrict__ posrho,
+                                                            const float* __restrict__ q, const DevGrid* g,
+                                                            const uint32_t* __restrict__ cell_lb, SimConst c,
+                                                            const float* __restrict__ tables,
+                                                            float4* __restrict__ force, uint32_t nlogical) {
+    if (step_dead(g)) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sP1[LPS_TABLE_SIZE];
+    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
+    __syncthreads();
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const GridView G = grid_view(g);
+    const float4 xi = posrho[i];
+    const float tf = q[i];
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f;
+    for_neighbors(G, cell_lb, xi.x, xi.y, xi.z, [&](uint32_t j) {
+        const float4 xj = posrho[j];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float s2 = dx * dx + dy * dy + dz * dz;
+        if (s2 <= c.s_max && i != j) {
+            const float d = sqrtf(s2);
+            const float ux = (xi.x - xj.x) / d, uy = (xi.y - xj.y) / d, uz = (xi.z - xj.z) / d;
+            const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
+            const float sc = tf + q[j];
+            pfx += ux * sc * w;
+            pfy += uy * sc * w;
+            pfz += uz * sc * w;
+        }
+    });
+    float4 f = force[i];
+    f.x += (-pfx) * c.mass * xi.w;
+    f.y += (-pfy) * c.mass * xi.w;
+    f.z += (-pfz) * c.mass * xi.w;
+    force[i] = f;
+}
+
+// pciCalcDensityPressure (SPHKernels.cl:737-804) + the density-error maximum
+// of iteration `slot` (max_i |e_i|, e_i = rho*_i - rho0 scaled by npf when the
+// pressure update is negative; PCISPHSolver.cpp:113-121,152-156).  Cells come
+// from the current position, distances from the predicted ones.
+__global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
+    const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
+    SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
+    float* __restrict__ q, uint32_t first_iter, uint32_t slot, uint32_t nlogical) {
+    if (step_dead(g) || g->done) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sW[LPS_TABLE_SIZE];
+    __shared__ uint32_t s_max[LPS_BLOCK / 64];
+    load_table(sW, tables);
+    __syncthreads();
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    uint32_t ebits = 0;
+    if (i < c.n) {
+        const GridView G = grid_view(g);
+        const float4 xa = pos[i];
+        const float4 xi = xpred[i];
+        float sum = 0.f;
+        for_neighbors(G, cell_lb, xa.x, xa.y, xa.z, [&](uint32_t j) {
+            const float4 xj = xpred[j];
+            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            const float s2 = dx * dx + dy * dy + dz * dz;
+            if (s2 <= c.s_max) {
+                const float d = sqrtf(s2);
+                sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
+            }
+        });
+        const float rp = c.mass * sum;
+        float e = rp - c.rest_density;
+        float dp = c.delta * e;
+        if (dp < 0.f) {
+            dp *= c.npf;
+            e *= c.npf;
+        }
+        const float p = (first_iter ? 0.f : press[i]) + dp;
+        rho_pred[i] = rp;
+        press[i] = p;
+        q[i] = p / (rp * rp);
+        ebits = __float_as_uint(fabsf(e));
+    }
+    ebits = wave_max_u32(ebits);
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = ebits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) m = max(m, s_max[k]);
+        if (m) atomicMax(&g->err_bits[slot], m);
+    }
+}
+
+// pciCalcPressureForce (SPHKernels.cl:806-878) followed by the next
+// prediction (pciIntegrate + pciHandleCollisions) from f + fp.
+__global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
+    const float4* __restrict__ pos, const float4* __restrict__ xpred, const float* __restrict__ rho_pred,
+    const float* __restrict__ q, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c,
+    const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
+    const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
+    const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres,
+    const lps_plane* __restrict__ planes, float4* __restrict__ xpred_next, uint32_t nlogical) {
+    if (step_dead(g) || g->done) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sP1[LPS_TABLE_SIZE];
+    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
+    __syncthreads();
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    if (i >= c.n) return;
+    const GridView G = grid_view(g);
+    const float4 xa = pos[i];
+    const float4 xi = xpred[i];
+    const float tf = q[i];
+    float pfx = 0.f, pfy = 0.f, pfz = 0.f;
+    for_neighbors(G, cell_lb, xa.x, xa.y, xa.z, [&](uint32_t j) {
+        const float4 xj = xpred[j];
+        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+        const float s2 = dx * dx + dy * dy + dz * dz;
+        if (s2 <= c.s_max && i != j) {
+            const float d = sqrtf(s2);
+            const float ux = (xi.x - xj.x) / d, uy = (xi.y - xj.y) / d, uz = (xi.z - xj.z) / d;
+            const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
+            const float sc = tf + q[j];
+            pfx += ux * sc * w;
+            pfy += uy * sc * w;
+            pfz += uz * sc * w;
+        }
+    });
+    const float ri = rho_pred[i];
+    const float4 f4 = make_float4((-pfx) * c.mass * ri, (-pfy) * c.mass * ri, (-pfz) * c.mass * ri, 0.f);
+    fp[i] = f4;
+    const float4 f = force[i];
+    const float4 xr = posrho[i];
+    xpred_next[i] = predict(make_float4(xr.x, xr.y, xr.z, 0.f), vel[i], vh[i], first[i] != 0, f.x + f4.x,
+                            f.y + f4.y, f.z + f4.z, xr.w, c, boxes, spheres, planes);
+}
+
+// Convergence of iteration k (1-based): stop when k >= min and the ratio is
+// below the tolerance; a NaN error marks the step non-finite.
+__global__ void k_pci_check(DevGrid* g, uint32_t k, uint32_t slot, uint32_t min_iter, float rest_density,
+                            float tol, uint32_t* __restrict__ host_flag) {
+    if (threadIdx.x != 0) return;
+    uint32_t state = 0;  // 1: converged / stopped, 2: error
+    if (g->status != 0u) {
+        state = 2;
+    } else if (g->done) {
+        state = 1;
+    } else {
+        const uint32_t eb = g->err_bits[slot];
+        if (eb >= 0x7F800000u) {
+            g->status = LPS_ERR_NONFINITE;
+            state = 2;
+        } else {
+            const float ratio = __uint_as_float(eb) / rest_density;
+            if (k >= min_iter && ratio < tol) state = 1;
+        }
+    }
+    if (state == 1u && !g->done) {
+        g->done = 1u;
+        g->iterations = k;
+    }
+    host_flag[0] = state;
+    host_flag[1] = g->done ? g->iterations : k;
+    host_flag[2] = g->err_bits[slot];
+}
+
+// integrate (SPHKernels.cl:424-455) + pciAddPressureForce (SPHKernels.cl:880-890)
+// + handleCollisions (SPHKernels.cl:457-486); accumulates the bounding box of
+// the new positions for the next step.
+__global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
+    float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ vh, uint8_t* __restrict__ first,
+    const float4* __restrict__ force, const float4* __restrict__ fp, const float4* __restrict__ posrho,
+    float* __restrict__ pprev, const float* __restrict__ press, DevGrid* g, SimConst c,
+    const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres, const lps_plane* __restrict__ planes,
+    uint32_t next_slot, uint32_t nlogical) {
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    const bool dead = step_dead(g);
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    const bool valid = i < c.n;
+    float3 p = make_float3(0.f, 0.f, 0.f);
+    if (valid) {
+        const float4 x4 = pos[i];
+        p = make_float3(x4.x, x4.y, x4.z);
+        if (!dead) {
+            float4 f = force[i];
+            if (fp) {
+                const float4 a = fp[i];
+                f.x += a.x; f.y += a.y; f.z += a.z;
+            }
+            const float rho = posrho[i].w;
+            const float ax = f.x / rho, ay = f.y / rho, az = f.z / rho;
+            const float4 h4 = vh[i];
+            float3 hv = make_float3(h4.x, h4.y, h4.z);
+            if (first[i]) {
+                const float4 v4 = vel[i];
+                hv.x = v4.x - ax * c.dt / 2.f;
+                hv.y = v4.y - ay * c.dt / 2.f;
+                hv.z = v4.z - az * c.dt / 2.f;
+                first[i] = 0;
+            }
+            const float3 old = hv;
+            hv.x += ax * c.dt; hv.y += ay * c.dt; hv.z += az * c.dt;
+            p.x += hv.x * c.dt; p.y += hv.y * c.dt; p.z += hv.z * c.dt;
+            collide_all(p, hv, c, boxes, spheres, planes);
+            vh[i] = make_float4(hv.x, hv.y, hv.z, 0.f);
+            pos[i] = make_float4(p.x, p.y, p.z, 0.f);
+            vel[i] = make_float4((old.x + hv.x) / 2.f, (old.y + hv.y) / 2.f, (old.z + hv.z) / 2.f, 0.f);
+            if (pprev) pprev[i] = press[i];
+        }
+    }
+    bbox_accumulate(g, next_slot, valid, p.x, p.y, p.z);
+}
+
+// ====================================================== host launchers ====
+namespace lps_dev {
+
+static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
+static inline uint32_t xcd_grid(uint32_t nlogical) { return cdiv(nlogical, LPS_XCDS) * LPS_XCDS; }
+
+void launch_bbox_reset(hipStream_t s, DevGrid* g, uint32_t slot) { k_bbox_reset<<<1, 64, 0, s>>>(g, slot); }
+
+void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, DevGrid* g, uint32_t slot) {
+    const uint32_t nl = cdiv(n, LPS_BLOCK);
+    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, n, g, slot, nl);
+}
+
+void launch_grid_setup(hipStream_t s, DevGrid* g, uint32_t slot, float h, uint32_t cell_cap) {
+    k_grid_setup<<<1, 64, 0, s>>>(g, slot, h, cell_cap);
+}
+
+void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, uint32_t* keys, uint32_t* vals) {
+    k_keys<<<cdiv(n, LPS_BLOCK), LPS_BLOCK, 0, s>>>(pos, n, g, keys, vals);
+}
+
+uint32_t radix_tiles(uint32_t n) { return cdiv(n, LPS_RADIX_TILE); }
+
+void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
+                       uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist) {
+    const uint32_t nt = radix_tiles(n);
+    k_radix_hist<<<nt, LPS_BLOCK, 0, s>>>(kin, n, shift, g, hist, nt);
+    k_radix_scan<<<1, 1024, 0, s>>>(hist, nt * 256u, g, shift);
+    k_radix_scatter<<<nt, LPS_BLOCK, 0, s>>>(kin, vin, kout, vout, n, shift, g, hist, nt);
+}
+
+void launch_reorder(hipStream_t s, const ReorderArgs& a) {
+    const uint32_t nl = cdiv(a.n, LPS_BLOCK);
+    k_reorder<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.keysA, a.valsA, a.keysB, a.valsB, a.n, a.g, a.pos_in, a.vel_in,
+                                                 a.vh_in, a.first_in, a.id_in, a.pprev_in, a.pos_out, a.vel_out,
+                                                 a.vh_out, a.first_out, a.id_out, a.pprev_out, a.run_end,
+                                                 a.skeys, a.perm, nl);
+}
+
+uint32_t cell_blocks(uint32_t cell_cap) { return cdiv(cell_cap + 1u, LPS_SCAN_TILE); }
+
+void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint32_t* block_max, uint32_t* cell_lb,
+                       uint32_t cell_cap) {
+    const uint32_t nb = cell_blocks(cell_cap);
+    k_cell_reduce<<<nb, LPS_BLOCK, 0, s>>>(run_end, g, block_max);
+    k_cell_scan<<<nb, LPS_BLOCK, 0, s>>>(run_end, g, block_max, cell_lb);
+}
+
+void launch_density(hipStream_t s, const float4* pos, const DevGrid* g, const uint32_t* cell_lb, const SimConst& c,
+                    const float* tables, float4* posrho, float* press, float* q) {
+    const uint32_t nl = cdiv(c.n, LPS_BLOCK);
+    k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, g, cell_lb, c, tables, posrho, press, q, nl);
+}
+
+void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_forces<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables,
+                                                a.boxes, a.spheres, a.planes, a.force, xpred, nl);
+}
+
+void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_pressure_sph<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, nl);
+}
+
+void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
+                        float* q, bool first_iter, uint32_t slot) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_pci_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q,
+                                                     first_iter ? 1u : 0u, slot, nl);
+}
+
+void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
+                         const float* q, float4* fp, float4* xpred_next) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_pci_pressure<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp,
+                                                      a.force, a.posrho, a.vel, a.vh, a.first, a.boxes, a.spheres,
+                                                      a.planes, xpred_next, nl);
+}
+
+void launch_pci_check(hipStream_t s, DevGrid* g, uint32_t k, uint32_t slot, uint32_t min_iter, float rest_density,
+                      float tol, uint32_t* host_flag) {
+    k_pci_check<<<1, 64, 0, s>>>(g, k, slot, min_iter, rest_density, tol, host_flag);
+}
+
+void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
+                      uint32_t next_slot) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_integrate<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.vel, a.vh, a.first, a.force, fp, a.posrho, pprev, press,
+                                                   a.g, a.c, a.boxes, a.spheres, a.planes, next_slot, nl);
+}
+
+}  // namespace lps_dev

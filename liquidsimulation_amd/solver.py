@@ -1,0 +1,304 @@
+"""Host-side mirror of the reference solver interface over the C ABI.
+
+The reference's scene + step API is the C++ class pair SPHSolver /
+PCISPHSolver (LiquidPhysics/include/SPHSolver.h:25-69, PCISPHSolver.h:7-20):
+particles and static colliders are added, parameters are set through setters,
+and PhysicSolver::update(dt) advances one step (PhysicSolver.cpp:12-21).  The
+classes below keep those method names and meanings, with particles held as
+arrays (the library keeps the state on the device, in the reference's sorted
+order, like its OpenCL path; SPHSolver.cpp:412-422) instead of SPHParticle
+objects.  Every call goes through liblps.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+from .abi import (CBox, CGridInfo, CParams, CPlane, CSphere, CStepStats, Box, Colliders, Params, Plane,
+                  Sphere, LPS_TABLE_SIZE, LPS_MLP_IN, LPS_MLP_PARAMS, METHOD_PCISPH, METHOD_SPH)
+from ._lib import check, lib, ptr
+
+
+def device_info():
+    name = C.create_string_buffer(256)
+    cus = C.c_int32()
+    st = lib().lps_device_info(name, 256, C.byref(cus))
+    check(st, "lps_device_info")
+    return name.value.decode(), cus.value
+
+
+class SPHSolver:
+    """SPH solver backed by the gfx950 library (reference: SPHSolver)."""
+
+    _default_method = METHOD_SPH
+
+    def __init__(self, params: Optional[Params] = None):
+        self._params = params if params is not None else Params(method=self._default_method)
+        self._ctx = C.c_void_p()
+        cp = self._params.to_c()
+        check(lib().lps_create(C.byref(cp), C.byref(self._ctx)), "lps_create")
+        self._colliders = Colliders()
+        self._pending: List[np.ndarray] = []
+        self._n = 0
+        self.last_stats: List[CStepStats] = []
+
+    # ---- lifetime
+    def close(self):
+        if self._ctx:
+            lib().lps_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- parameters (SPHSolver setters, SPHSolver.cpp:628-710)
+    @property
+    def params(self) -> Params:
+        return self._params
+
+    def _apply(self, **kw):
+        self._params = self._params.replace(**kw)
+        cp = self._params.to_c()
+        check(lib().lps_set_params(self._ctx, C.byref(cp)), "lps_set_params")
+
+    def set_params(self, params: Params):
+        self._params = params
+        cp = params.to_c()
+        check(lib().lps_set_params(self._ctx, C.byref(cp)), "lps_set_params")
+
+    def setGravity(self, g):
+        self._apply(gravity=tuple(float(v) for v in g))
+
+    def setParticleRadius(self, r):
+        self._apply(particle_radius=float(r))
+
+    def setKernelRadiusFactor(self, f):
+        self._apply(kernel_radius_factor=float(f))
+
+    def setRestDensity(self, v):
+        self._apply(rest_density=float(v))
+
+    def setViscosityCoefficient(self, v):
+        self._apply(viscosity=float(v))
+
+    def setPressureStiffnessCoefficient(self, v):
+        self._apply(stiffness=float(v))
+
+    def setNegativePressureFactor(self, v):
+        self._apply(negative_pressure_factor=float(v))
+
+    def setSurfaceTensionCoefficient(self, v):
+        self._apply(surface_tension=float(v))
+
+    def setSurfaceTensionThreshold(self, v):
+        self._apply(surface_tension_threshold=float(v))
+
+    def setRestitutionCoefficient(self, v):
+        self._apply(restitution=float(v))
+
+    def setFrictionCoefficient(self, v):
+        self._apply(friction=float(v))
+
+    # ---- getters
+    def getParticleCount(self) -> int:
+        return self._n
+
+    def getParticleRadius(self):
+        return self._params.particle_radius
+
+    def _derived(self):
+        h, ds, m = C.c_float(), C.c_float(), C.c_float()
+        t = np.zeros(5 * LPS_TABLE_SIZE, np.float32)
+        check(lib().lps_get_derived(self._ctx, C.byref(h), C.byref(ds), C.byref(m), t.ctypes.data),
+              "lps_get_derived")
+        return h.value, ds.value, m.value, t.reshape(5, LPS_TABLE_SIZE)
+
+    def getKernelRadius(self):
+        return self._derived()[0]
+
+    def getParticleMass(self):
+        return self._derived()[2]
+
+    def kernel_tables(self):
+        return self._derived()[3]
+
+    def division_step(self):
+        return self._derived()[1]
+
+    # ---- scene (addParticle / addStaticCollisionObject / cleanUp)
+    def set_particles(self, x, v=None, vhalf=None, first=None):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        n = x.shape[0]
+        v = None if v is None else np.ascontiguousarray(v, np.float32).reshape(n, 3)
+        vhalf = None if vhalf is None else np.ascontiguousarray(vhalf, np.float32).reshape(n, 3)
+        first = None if first is None else np.ascontiguousarray(first, np.uint8).reshape(n)
+        self._push_colliders()
+        check(lib().lps_set_particles(self._ctx, x.reshape(-1), ptr(v), ptr(vhalf), ptr(first), n),
+              "lps_set_particles")
+        self._n = n
+        self._pending = []
+
+    def addParticles(self, x, v=None):
+        """Queue particles (positions, velocities); uploaded at the next update."""
+        x = np.asarray(x, np.float32).reshape(-1, 3)
+        v = np.zeros_like(x) if v is None else np.asarray(v, np.float32).reshape(-1, 3)
+        self._pending.append((x, v))
+
+    def removeParticles(self):
+        self._pending = []
+        self._n = 0
+
+    def addStaticCollisionObject(self, obj):
+        if isinstance(obj, Box):
+            self._colliders.boxes.append(obj)
+        elif isinstance(obj, Sphere):
+            self._colliders.spheres.append(obj)
+        elif isinstance(obj, Plane):
+            self._colliders.planes.append(obj)
+        else:
+            raise TypeError(type(obj))
+        self._colliders_dirty = True
+
+    def set_colliders(self, col: Colliders):
+        self._colliders = col
+        self._colliders_dirty = True
+        self._push_colliders()
+
+    def removeStaticCollisionObjects(self):
+        self._colliders = Colliders()
+        self._colliders_dirty = True
+
+    def cleanUp(self):
+        self.removeParticles()
+        self.removeStaticCollisionObjects()
+
+    def _push_colliders(self):
+        if not getattr(self, "_colliders_dirty", True):
+            return
+        boxes, nb, spheres, ns, planes, npl = self._colliders.c_arrays()
+        check(lib().lps_set_colliders(self._ctx, C.cast(boxes, C.c_void_p), nb, C.cast(spheres, C.c_void_p), ns,
+                                      C.cast(planes, C.c_void_p), npl), "lps_set_colliders")
+        self._colliders_dirty = False
+
+    def _flush_pending(self):
+        if self._pending:
+            old = self.get_particles() if self._n else None
+            xs = [p[0] for p in self._pending]
+            vs = [p[1] for p in self._pending]
+            if old is not None:
+                xs.insert(0, old["x"])
+                vs.insert(0, old["v"])
+            x = np.concatenate(xs)
+            v = np.concatenate(vs)
+            first = np.ones(x.shape[0], np.uint8)
+            vh = np.zeros_like(x)
+            if old is not None:
+                first[: self._n] = old["first"]
+                vh[: self._n] = old["vhalf"]
+            self.set_particles(x, v, vh, first)
+        self._push_colliders()
+
+    # ---- step (PhysicSolver::update)
+    def update(self, dt: float, nsteps: int = 1):
+        self._flush_pending()
+        stats = (CStepStats * max(nsteps, 1))()
+        st = lib().lps_step(self._ctx, float(dt), int(nsteps), C.cast(stats, C.c_void_p))
+        self.last_stats = list(stats[:nsteps])
+        check(st, "lps_step")
+        return self.last_stats
+
+    # ---- readback (onEndUpdate) and parity hooks
+    def get_particles(self):
+        n = self._n
+        x = np.zeros((n, 3), np.float32)
+        v = np.zeros((n, 3), np.float32)
+        vh = np.zeros((n, 3), np.float32)
+        first = np.zeros(n, np.uint8)
+        ids = np.zeros(n, np.uint32)
+        check(lib().lps_get_particles(self._ctx, ptr(x), ptr(v), ptr(vh), ptr(first), ptr(ids), n),
+              "lps_get_particles")
+        return {"x": x, "v": v, "vhalf": vh, "first": first, "id": ids}
+
+    def get_grid(self, cell_cap: Optional[int] = None):
+        info = CGridInfo()
+        check(lib().lps_get_grid(self._ctx, C.byref(info), None, None, None, None, 0), "lps_get_grid")
+        cells = info.cell_count if cell_cap is None else min(cell_cap, info.cell_count)
+        n = self._n
+        keys = np.zeros(n, np.uint32)
+        perm = np.zeros(n, np.uint32)
+        cs = np.zeros(max(cells, 1), np.uint32)
+        ce = np.zeros(max(cells, 1), np.uint32)
+        check(lib().lps_get_grid(self._ctx, C.byref(info), ptr(keys), ptr(perm), ptr(cs), ptr(ce), cells),
+              "lps_get_grid")
+        return {
+            "bbox_min": np.array(info.bbox_min[:], np.float32),
+            "bbox_max": np.array(info.bbox_max[:], np.float32),
+            "grid_offset": np.array(info.grid_offset[:], np.float32),
+            "grid_spacing": info.grid_spacing,
+            "grid_size": np.array(info.grid_size[:], np.uint32),
+            "cell_count": info.cell_count,
+            "keys": keys, "perm": perm, "cell_start": cs[:cells], "cell_end": ce[:cells],
+        }
+
+    def get_fields(self):
+        n = self._n
+        rho = np.zeros(n, np.float32)
+        p = np.zeros(n, np.float32)
+        f = np.zeros((n, 3), np.float32)
+        rp = np.zeros(n, np.float32)
+        check(lib().lps_get_fields(self._ctx, ptr(rho), ptr(p), ptr(f), ptr(rp)), "lps_get_fields")
+        return {"rho": rho, "pressure": p, "force": f, "rho_pred": rp}
+
+    def delta(self, dt: float) -> float:
+        d = C.c_float()
+        check(lib().lps_get_delta(self._ctx, float(dt), C.byref(d)), "lps_get_delta")
+        return d.value
+
+    # ---- timing
+    def profile(self, on: bool = True):
+        check(lib().lps_profile_enable(self._ctx, 1 if on else 0), "lps_profile_enable")
+
+    def profile_reset(self):
+        check(lib().lps_profile_reset(self._ctx), "lps_profile_reset")
+
+    def profile_query(self, name: str):
+        n = C.c_uint32()
+        ms = C.c_double()
+        check(lib().lps_profile_query(self._ctx, name.encode(), C.byref(n), C.byref(ms)), "lps_profile_query")
+        return n.value, ms.value
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(lib().lps_get_stream(self._ctx, C.byref(s)), "lps_get_stream")
+        return s.value or 0
+
+
+class PCISPHSolver(SPHSolver):
+    """PCISPH solver (reference: PCISPHSolver, PCISPHSolver.cpp:50-240)."""
+
+    _default_method = METHOD_PCISPH
+
+    def setMinIterations(self, k: int):
+        self._apply(min_iterations=int(k))
+
+    def setMaxIterations(self, k: int):
+        self._apply(max_iterations=int(k))
+
+    def setMaxDensityErrorRatio(self, r: float):
+        self._apply(max_density_error_ratio=float(r))
+
+    def getMinIterations(self):
+        return self._params.min_iterations
+
+    def getMaxDensityErrorRatio(self):
+        return self._params.max_density_error_ratio
+
+
+def solver_for(params: Params):
+    return PCISPHSolver(params) if params.method == METHOD_PCISPH else SPHSolver(params)
