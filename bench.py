@@ -1,0 +1,192 @@
+"""Benchmark: particle-steps/s of the PCISPH dam break on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Workload (default): the 524,288-particle PCISPH dam break (64x128x64 block at
+0.00625 m in a 1.6x1.0x0.8 m box, dt 0.001 s, 1 % density-error tolerance,
+3..50 corrections, +-1e-5 m seeded jitter).  A step is one PhysicSolver::update
+of the whole scene; the state stays resident in HBM during the timed region.
+value = particles * K / wall seconds of the K timed steps (HIP events on the
+library's stream, bracketed by device synchronisation).
+
+Prints ONE JSON line.  Extra keys: per-step PCISPH iteration statistics,
+convergence qualification, the dominant kernel's roofline (algorithmic bytes
+per launch over its event-timed average duration) and the CPU baseline (the
+oracle's OpenMP restatement on this host, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) bytes per particle for each kernel: every particle's
+# own state read or written once, FP32 SoA (vec3 = 12 B, scalar = 4 B);
+# neighbour re-reads not counted.  Per PCISPH correction 108 B = predicted
+# density + pressure update + error max (24 B) + pressure force + re-prediction
+# (84 B); fixed part 348 B at three radix passes.
+ALGO_BYTES = {
+    "pci_density": 24.0,
+    "pci_pressure": 84.0,
+    "density": 16.0,
+    "forces": 40.0 + 52.0,   # non-pressure forces + first prediction
+    "pressure_sph": 36.0,
+    "integrate": 88.0,
+}
+
+
+def step_bytes(k: float, passes: int = 3) -> float:
+    """Compulsory bytes per particle-step: 300 + 16P fixed + 108 per correction."""
+    return 300.0 + 16.0 * passes + 108.0 * k
+
+
+def run_gpu(scene, steps: int, warmup: int, profile_window: int):
+    import torch
+    from liquidsimulation_amd.solver import solver_for
+
+    s = solver_for(scene.params)
+    s.set_colliders(scene.colliders)
+    s.set_particles(scene.x, scene.v)
+    if warmup:
+        s.update(scene.dt, warmup)
+    stream = torch.cuda.ExternalStream(s.stream())
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    stats = s.update(scene.dt, steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1)
+    its = np.array([x.iterations for x in stats], np.int64)
+    errs = np.array([x.max_density_error_ratio for x in stats], np.float64)
+    conv = np.array([x.converged for x in stats], np.int64)
+    # dominant kernel: per-launch HIP events on the library stream over a window
+    # of steps continuing from the timed state
+    s.profile(True)
+    s.profile_reset()
+    win = s.update(scene.dt, profile_window)
+    kernels = {}
+    for name in ("grid_setup", "keys", "radix_pass", "reorder", "cell_table", "density", "forces",
+                 "pressure_sph", "pci_density", "pci_pressure", "integrate"):
+        n, tot = s.profile_query(name)
+        if n:
+            kernels[name] = {"launches": n, "total_ms": tot, "avg_us": 1000.0 * tot / n}
+    s.profile(False)
+    win_its = np.array([x.iterations for x in win], np.int64)
+    s.close()
+    return {"ms": ms, "wall": wall, "its": its, "errs": errs, "conv": conv, "kernels": kernels,
+            "win_its": win_its}
+
+
+def cpu_baseline(scene, warm: int, timed: int):
+    from oracle import pyoracle as O
+    nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    nthreads = max(1, min(nthreads, 64))
+    st = O.State.from_scene(scene.x, scene.v)
+    if warm:
+        O.run_ocl(scene.params, scene.colliders, scene.dt, warm, st, nthreads)
+    t0 = time.perf_counter()
+    status, its, errs = O.run_ocl(scene.params, scene.colliders, scene.dt, timed, st, nthreads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": scene.n * timed / dt,
+        "unit": "particle-steps/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"{warm} warm-up + {timed} timed steps of {scene.name} from its initial state "
+                  f"(OpenMP restatement of the OpenCL-kernel path, {nthreads} threads, "
+                  f"mean {float(np.mean(its)):.2f} corrections/step)",
+        "seconds": dt,
+        "status": int(status),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="pcisph_dam_break_524k")
+    ap.add_argument("--profile-window", type=int, default=100)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-warm", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only (the SPH step has no multi-GPU path); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from liquidsimulation_amd import scenes
+    scene = scenes.WORKLOADS[args.workload]()
+    res = run_gpu(scene, args.steps, args.warmup, args.profile_window)
+    n = scene.n
+    value = n * args.steps / (res["ms"] / 1000.0)
+    its, errs = res["its"], res["errs"]
+    tol = scene.params.max_density_error_ratio
+    nonconv = int(np.sum(res["conv"] == 0)) if scene.params.method == 1 else 0
+    kernels = res["kernels"]
+    dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])[0] if kernels else None
+    roof = None
+    if dom:
+        per = ALGO_BYTES.get(dom)
+        avg_s = kernels[dom]["avg_us"] * 1e-6
+        if per:
+            achieved = per * n / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                    "algo_bytes_per_launch": per * n, "avg_launch_us": kernels[dom]["avg_us"]}
+    mean_k = float(np.mean(its)) if its.size else 0.0
+    step_roof = n * step_bytes(mean_k) * args.steps / (res["ms"] / 1000.0) / 1e9
+    out = {
+        "metric": "particle-steps/s at <=1% density error (plain vs learned warm start), 1 GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded dam-break lattice with jitter)",
+        "config": {"workload": scene.name, "particles": n, "dt": scene.dt,
+                   "method": "PCISPH" if scene.params.method == 1 else "SPH",
+                   "tolerance": tol, "min_iterations": scene.params.min_iterations,
+                   "max_iterations": scene.params.max_iterations,
+                   "stall_iterations": scene.params.stall_iterations, "warm_start": "none"},
+        "iterations": {"mean": mean_k, "p50": float(np.percentile(its, 50)) if its.size else 0,
+                       "p99": float(np.percentile(its, 99)) if its.size else 0,
+                       "max": int(its.max()) if its.size else 0},
+        "qualified": nonconv == 0,
+        "nonconverged_steps": nonconv,
+        "worst_density_error": float(errs.max()) if errs.size else 0.0,
+        "roofline": roof,
+        "step_roofline": {"achieved_GBps": step_roof, "frac": step_roof / HBM_PEAK_GBS,
+                          "bytes_per_particle_step": step_bytes(mean_k)},
+        "kernels": kernels,
+        "wall_s": res["wall"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_warm, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -72,6 +72,9 @@ typedef struct lps_params {
     int32_t max_iterations;          /* PCISPH cap */
     float max_density_error_ratio;   /* setMaxDensityErrorRatio: stop when max|e|/rho0 < this */
     int32_t warm_start;              /* lps_warm_start */
+    int32_t stall_iterations;        /* also stop once the error has not improved on its best value
+                                        for this many corrections (0: off); such steps are reported
+                                        as not converged */
 } lps_params;
 
 /* Static colliders (StaticCollisionBox / StaticCollisionSphere, include/Collision/),
@@ -85,6 +88,7 @@ typedef struct lps_step_stats {
     float max_density_error_ratio;   /* max_i |e_i| / rho0 after the last correction */
     int32_t warm_start_used;         /* 1 if the learned seed was applied */
     int32_t status;                  /* lps_status of this step */
+    int32_t converged;               /* 1: tolerance met; 0: stopped at the cap or by the stall rule */
 } lps_step_stats;
 
 typedef struct lps_grid_info {

@@ -53,6 +53,7 @@ class CParams(C.Structure):
         ("max_iterations", C.c_int32),
         ("max_density_error_ratio", C.c_float),
         ("warm_start", C.c_int32),
+        ("stall_iterations", C.c_int32),
     ]
 
 
@@ -74,6 +75,7 @@ class CStepStats(C.Structure):
         ("max_density_error_ratio", C.c_float),
         ("warm_start_used", C.c_int32),
         ("status", C.c_int32),
+        ("converged", C.c_int32),
     ]
 
 
@@ -110,6 +112,7 @@ class Params:
     max_iterations: int = 4
     max_density_error_ratio: float = 0.1
     warm_start: int = WARM_NONE
+    stall_iterations: int = 0
 
     def to_c(self) -> CParams:
         c = CParams()
@@ -125,6 +128,7 @@ class Params:
         c.min_iterations = int(self.min_iterations)
         c.max_iterations = int(self.max_iterations)
         c.warm_start = int(self.warm_start)
+        c.stall_iterations = int(self.stall_iterations)
         return c
 
     def replace(self, **kw) -> "Params":

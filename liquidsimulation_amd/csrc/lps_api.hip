@@ -43,7 +43,6 @@ struct lps_ctx {
     uint32_t cell_cap = 0;  // allocated cell-table entries (plus one sentinel)
     uint32_t passes = 0;    // radix passes for cell_cap
     int cur = 0;            // which state buffer holds the sorted state
-    uint32_t parity = 0;    // bbox accumulator read by the next step
     uint64_t steps = 0;
     bool have_step = false; // a step has produced grid/fields
 
@@ -55,6 +54,8 @@ struct lps_ctx {
     uint32_t *keysA = nullptr, *valsA = nullptr, *keysB = nullptr, *valsB = nullptr;
     uint32_t *skeys = nullptr, *perm = nullptr;
     uint32_t* hist = nullptr;
+    uint32_t* digit_total = nullptr;
+    uint32_t* bbox_part = nullptr;
     float4 *posrho = nullptr, *force = nullptr, *fp = nullptr, *xpred[2] = {};
     float *press = nullptr, *q = nullptr, *rho_pred = nullptr;
     uint32_t *run_end = nullptr, *cell_lb = nullptr, *block_max = nullptr;
@@ -264,6 +265,8 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if ((st = dalloc(&c->skeys, n))) return st;
     if ((st = dalloc(&c->perm, n))) return st;
     if ((st = dalloc(&c->hist, (size_t)256 * lps_dev::radix_tiles(n)))) return st;
+    if ((st = dalloc(&c->digit_total, 256))) return st;
+    if ((st = dalloc(&c->bbox_part, (size_t)LPS_BBOX_PART * lps_dev::bbox_parts(n)))) return st;
     if ((st = dalloc(&c->posrho, n))) return st;
     if ((st = dalloc(&c->force, n))) return st;
     if ((st = dalloc(&c->fp, n))) return st;
@@ -280,7 +283,7 @@ void free_all(lps_ctx* c) {
         dfree(c->id[b]); dfree(c->pprev[b]); dfree(c->xpred[b]);
     }
     dfree(c->keysA); dfree(c->valsA); dfree(c->keysB); dfree(c->valsB);
-    dfree(c->skeys); dfree(c->perm); dfree(c->hist);
+    dfree(c->skeys); dfree(c->perm); dfree(c->hist); dfree(c->digit_total); dfree(c->bbox_part);
     dfree(c->posrho); dfree(c->force); dfree(c->fp); dfree(c->press); dfree(c->q); dfree(c->rho_pred);
     dfree(c->run_end); dfree(c->cell_lb); dfree(c->block_max);
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
@@ -316,7 +319,7 @@ void enqueue_grid(lps_ctx* c) {
     const uint32_t n = c->n;
     {
         Timed t(c, "grid_setup");
-        lps_dev::launch_grid_setup(s, c->grid, c->parity, c->der.h, c->cell_cap);
+        lps_dev::launch_grid_setup(s, c->grid, c->bbox_part, n, c->der.h, c->cell_cap);
     }
     {
         Timed t(c, "keys");
@@ -327,7 +330,7 @@ void enqueue_grid(lps_ctx* c) {
         const bool even = (p & 1u) == 0u;
         lps_dev::launch_radix_pass(s, even ? c->keysA : c->keysB, even ? c->valsA : c->valsB,
                                    even ? c->keysB : c->keysA, even ? c->valsB : c->valsA, n, 8u * p, c->grid,
-                                   c->hist);
+                                   c->hist, c->digit_total);
     }
     {
         Timed t(c, "reorder");
@@ -355,6 +358,7 @@ void enqueue_grid(lps_ctx* c) {
 struct StepResult {
     int32_t iterations = 0;
     uint32_t err_bits = 0;
+    int32_t converged = 1;
 };
 
 StepResult enqueue_step(lps_ctx* c, float dt) {
@@ -380,9 +384,8 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
         }
         {
             Timed t(c, "integrate");
-            lps_dev::launch_integrate(s, a, nullptr, c->pprev[c->cur], c->press, c->parity ^ 1u);
+            lps_dev::launch_integrate(s, a, nullptr, c->pprev[c->cur], c->press, c->bbox_part);
         }
-        c->parity ^= 1u;
         c->last_fp_valid = 0;
         return res;
     }
@@ -392,21 +395,24 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
     }
     const uint32_t kmin = (uint32_t)std::max(1, c->params.min_iterations);
     const uint32_t kmax = (uint32_t)std::max(c->params.max_iterations, (int32_t)kmin);
+    const LoopRule rule{kmin, kmax, (uint32_t)std::max(0, c->params.stall_iterations),
+                        c->params.max_density_error_ratio};
     auto enqueue_iter = [&](uint32_t kk) {
-        const uint32_t slot = std::min(kk - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
         {
             Timed t(c, "pci_density");
-            lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk == 1u, slot);
+            lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule,
+                                        c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
         }
+        (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
         {
             Timed t(c, "pci_pressure");
             lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp,
-                                         c->xpred[kk & 1u]);
+                                         c->xpred[kk & 1u], kk);
         }
-        lps_dev::launch_pci_check(s, c->grid, kk, slot, kmin, c->params.rest_density,
-                                  c->params.max_density_error_ratio, c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
-        (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
     };
+    // Corrections 1..min always run; afterwards one correction stays queued
+    // ahead of the result being waited for.  Queued corrections past the stop
+    // return at once on the device.
     for (uint32_t kk = 1; kk <= kmin; kk++) enqueue_iter(kk);
     uint32_t k_enq = kmin, k_known = 0;
     while (true) {
@@ -417,18 +423,15 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
         const uint32_t state = f[0];
         res.iterations = (int32_t)f[1];
         res.err_bits = f[2];
+        res.converged = (int32_t)f[3];
         if (state != 0u) break;
         k_known = kw;
-        if (kw >= kmax) {
-            res.iterations = (int32_t)kmax;
-            break;
-        }
+        if (kw >= kmax) break;
     }
     {
         Timed t(c, "integrate");
-        lps_dev::launch_integrate(s, a, c->fp, c->pprev[c->cur], c->press, c->parity ^ 1u);
+        lps_dev::launch_integrate(s, a, c->fp, c->pprev[c->cur], c->press, c->bbox_part);
     }
-    c->parity ^= 1u;
     c->last_fp_valid = 1;
     return res;
 }
@@ -582,10 +585,8 @@ int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* v
     HIPC(hipMemcpy(c->id[0], ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
     HIPC(hipMemset(c->pprev[0], 0, sizeof(float) * n));
     c->n = n;
-    c->parity = 0;
     c->have_step = false;
-    lps_dev::launch_bbox_reset(c->stream, c->grid, 0);
-    lps_dev::launch_bbox(c->stream, c->pos[0], n, c->grid, 0);
+    lps_dev::launch_bbox(c->stream, c->pos[0], n, c->bbox_part);
     HIPC(hipStreamSynchronize(c->stream));
     return LPS_OK;
 }
@@ -642,6 +643,7 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
                 o.max_density_error_ratio = __builtin_bit_cast(float, r.err_bits) / c->params.rest_density;
                 o.warm_start_used = 0;
                 o.status = LPS_OK;
+                o.converged = r.converged;
             }
         }
         HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));

@@ -22,11 +22,7 @@
 // reduction (integrate of the previous step, or the bbox kernel) and by the
 // grid set-up kernel; read by every later kernel of the step.
 struct DevGrid {
-    // two bbox accumulators, alternating by step parity; orderable-uint encoded floats
-    uint32_t bmin[2][3];
-    uint32_t bmax[2][3];
-    uint32_t nonfinite[2];
-    // grid of the current step (SPHSolver.cpp:1031-1040)
+    // grid of the current step (SPHSolver.cpp:1011-1040)
     float bbox_min[3];
     float bbox_max[3];
     float offset[3];
@@ -37,11 +33,17 @@ struct DevGrid {
     uint32_t passes_done;    // radix passes executed this step
     uint32_t status;         // lps_status of the step; kernels no-op when non-zero
     uint32_t cells_needed;   // cell count requested when status == GRID_OVERFLOW
-    uint32_t err_bits[LPS_MAX_ITER_SLOTS];  // max |e_i| (float bits) per PCISPH iteration
-    uint32_t done;           // PCISPH loop converged (or stopped): later corrections no-op
-    uint32_t iterations;     // corrections run when done was set
+    uint32_t err_bits[LPS_MAX_ITER_SLOTS];  // max |e_i| (float bits) per PCISPH correction
+    uint32_t stop_iter;      // correction after which the loop stops (0: running)
     uint32_t steps_ok;       // steps whose grid set-up succeeded (monotone counter)
+    uint32_t best_bits;      // best (smallest) max |e| of this step's corrections so far
+    uint32_t stall;          // corrections since the best value last improved
+    uint32_t converged;      // the tolerance was met
+    uint32_t blocks_done;    // ticket counter of the last-block-done reduction
 };
+
+// Per-block partial bounding box: orderable-uint min xyz, max xyz, non-finite flag.
+#define LPS_BBOX_PART 8
 
 // Per-step constants passed by value (the role of the 128-byte
 // ParallelSPHParameters block, ParallelSPHStructs.h:17-40).

@@ -4,6 +4,12 @@
 
 #include "lps_common.h"
 
+// Loop rule of the PCISPH corrections, evaluated on the device.
+struct LoopRule {
+    uint32_t min_iter, max_iter, stall_limit;
+    float tol;
+};
+
 namespace lps_dev {
 
 struct ReorderArgs {
@@ -40,13 +46,13 @@ struct PhysArgs {
     SimConst c;
 };
 
-void launch_bbox_reset(hipStream_t s, DevGrid* g, uint32_t slot);
-void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, DevGrid* g, uint32_t slot);
-void launch_grid_setup(hipStream_t s, DevGrid* g, uint32_t slot, float h, uint32_t cell_cap);
+void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, uint32_t* part);
+uint32_t bbox_parts(uint32_t n);
+void launch_grid_setup(hipStream_t s, DevGrid* g, const uint32_t* part, uint32_t n, float h, uint32_t cell_cap);
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, uint32_t* keys, uint32_t* vals);
 uint32_t radix_tiles(uint32_t n);
 void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                       uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist);
+                       uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist, uint32_t* digit_total);
 void launch_reorder(hipStream_t s, const ReorderArgs& a);
 uint32_t cell_blocks(uint32_t cell_cap);
 void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint32_t* block_max, uint32_t* cell_lb,
@@ -56,12 +62,10 @@ void launch_density(hipStream_t s, const float4* pos, const DevGrid* g, const ui
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred);
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, bool first_iter, uint32_t slot);
+                        float* q, uint32_t k, const LoopRule& rule, uint32_t* host_flag);
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                         const float* q, float4* fp, float4* xpred_next);
-void launch_pci_check(hipStream_t s, DevGrid* g, uint32_t k, uint32_t slot, uint32_t min_iter, float rest_density,
-                      float tol, uint32_t* host_flag);
+                         const float* q, float4* fp, float4* xpred_next, uint32_t k);
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
-                      uint32_t next_slot);
+                      uint32_t* bbox_part);
 
 }  // namespace lps_dev

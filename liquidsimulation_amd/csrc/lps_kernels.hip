@@ -179,7 +179,9 @@ __device__ __forceinline__ void collide_sphere(float3& pos, float3& vel, float r
     const float d = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
     if (!s.boundary) {
         if ((R + r) - d > 0.f) {
-            const float3 n = make_float3((pos.x - c.x) / d, (pos.y - c.y) / d, (pos.z - c.z) / d);
+            // a particle exactly at the centre (0/0 in the reference) is pushed along +y
+            const float3 n = d > 0.f ? make_float3((pos.x - c.x) / d, (pos.y - c.y) / d, (pos.z - c.z) / d)
+                                     : make_float3(0.f, 1.f, 0.f);
             const float k = R + r;
             resolve(pos, vel, e, mu, n, make_float3(c.x + n.x * k, c.y + n.y * k, c.z + n.z * k));
         }
@@ -233,23 +235,40 @@ __device__ __forceinline__ float4 predict(float4 x, float4 v, float4 vh, bool fi
 
 __device__ __forceinline__ bool step_dead(const DevGrid* g) { return g->status != 0u; }
 
+// Pressure-force direction (x_i - x_j) / d (SPHKernels.cl:406,865).  Two
+// distinct particles at exactly the same point (d == 0; the box collider snaps
+// every particle crossing three walls onto one corner point) would give 0/0 in
+// the reference; the pair gets the antisymmetric unit direction +y for the
+// higher sorted index and -y for the lower one, which separates it.
+__device__ __forceinline__ void pair_dir(const float4& xi, const float4& xj, float d, uint32_t i, uint32_t j,
+                                         float& ux, float& uy, float& uz) {
+    if (d > 0.f) {
+        ux = (xi.x - xj.x) / d;
+        uy = (xi.y - xj.y) / d;
+        uz = (xi.z - xj.z) / d;
+    } else {
+        ux = 0.f;
+        uy = i > j ? 1.f : -1.f;
+        uz = 0.f;
+    }
+}
+
 }  // namespace
 
 // ================================================================ grid ====
-// Bounding box of the positions (SPHSolver.cpp:1011-1029): min starts at
-// FLT_MAX and max at FLT_MIN as in the reference; non-finite coordinates flag
-// the step.  Accumulates into slot `slot` of the two alternating accumulators.
-__device__ __forceinline__ void bbox_accumulate(DevGrid* g, uint32_t slot, bool valid, float x, float y, float z) {
+// Bounding box of the positions (SPHSolver.cpp:1011-1029).  Each block writes
+// its partial min/max (orderable-uint encoding) and a non-finite flag; the
+// grid set-up kernel reduces the partials, so there is no global atomic.
+__device__ __forceinline__ void bbox_partial(uint32_t* __restrict__ part, uint32_t lb, bool valid, float x,
+                                             float y, float z) {
     __shared__ uint32_t s_red[7][LPS_BLOCK / 64];
     uint32_t mn[3], mx[3], nf = 0;
     const float v[3] = {x, y, z};
     for (int a = 0; a < 3; a++) {
         const bool fin = valid && isfinite(v[a]);
         if (valid && !isfinite(v[a])) nf = 1;
-        mn[a] = fin ? f2o(v[a]) : 0xFFFFFFFFu;
-        mx[a] = fin ? f2o(v[a]) : 0u;
-        mn[a] = wave_min_u32(mn[a]);
-        mx[a] = wave_max_u32(mx[a]);
+        mn[a] = wave_min_u32(fin ? f2o(v[a]) : 0xFFFFFFFFu);
+        mx[a] = wave_max_u32(fin ? f2o(v[a]) : 0u);
     }
     nf = wave_max_u32(nf);
     const uint32_t w = threadIdx.x >> 6;
@@ -261,56 +280,59 @@ __device__ __forceinline__ void bbox_accumulate(DevGrid* g, uint32_t slot, bool 
         s_red[6][w] = nf;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < 3; a++) {
-            uint32_t lo = 0xFFFFFFFFu, hi = 0u;
-            for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) {
-                lo = min(lo, s_red[a][k]);
-                hi = max(hi, s_red[3 + a][k]);
-            }
-            if (lo != 0xFFFFFFFFu) atomicMin(&g->bmin[slot][a], lo);
-            if (hi != 0u) atomicMax(&g->bmax[slot][a], hi);
-        }
-        uint32_t any = 0;
-        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) any |= s_red[6][k];
-        if (any) atomicOr(&g->nonfinite[slot], 1u);
+    if (threadIdx.x < 7) {
+        const uint32_t a = threadIdx.x;
+        uint32_t r = (a < 3) ? 0xFFFFFFFFu : 0u;
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) r = (a < 3) ? min(r, s_red[a][k]) : max(r, s_red[a][k]);
+        part[lb * LPS_BBOX_PART + a] = r;
     }
 }
 
-__global__ void k_bbox_reset(DevGrid* g, uint32_t slot) {
-    if (threadIdx.x == 0) {
-        for (int a = 0; a < 3; a++) {
-            g->bmin[slot][a] = f2o(FLT_MAX);
-            g->bmax[slot][a] = f2o(FLT_MIN);
-        }
-        g->nonfinite[slot] = 0u;
-        g->status = 0u;
-    }
-}
-
-__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, uint32_t n, DevGrid* g,
-                                                    uint32_t slot, uint32_t nlogical) {
+__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, uint32_t n,
+                                                    uint32_t* __restrict__ part, uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
     if (i < n) p = pos[i];
-    bbox_accumulate(g, slot, i < n, p.x, p.y, p.z);
+    bbox_partial(part, lb, i < n, p.x, p.y, p.z);
 }
 
-// Grid parameters (SPHSolver.cpp:1031-1040), one thread.  Also resets the
-// per-step counters and the other bbox accumulator.
-__global__ void k_grid_setup(DevGrid* g, uint32_t slot, float h, uint32_t cell_cap) {
+// Grid parameters (SPHSolver.cpp:1031-1040).  min starts at FLT_MAX and max at
+// FLT_MIN as in the reference (SPHSolver.cpp:1013-1014).  Resets the per-step
+// counters.
+__global__ __launch_bounds__(LPS_BLOCK) void k_grid_setup(DevGrid* g, const uint32_t* __restrict__ part,
+                                                          uint32_t nparts, float h, uint32_t cell_cap) {
+    __shared__ uint32_t s_r[7][LPS_BLOCK / 64];
+    uint32_t r[7];
+    for (int a = 0; a < 3; a++) {
+        r[a] = f2o(FLT_MAX);
+        r[3 + a] = f2o(FLT_MIN);
+    }
+    r[6] = 0u;
+    for (uint32_t k = threadIdx.x; k < nparts; k += LPS_BLOCK) {
+        const uint32_t* q = part + k * LPS_BBOX_PART;
+        for (int a = 0; a < 3; a++) {
+            r[a] = min(r[a], q[a]);
+            r[3 + a] = max(r[3 + a], q[3 + a]);
+        }
+        r[6] |= q[6];
+    }
+    for (int a = 0; a < 7; a++) r[a] = (a < 3) ? wave_min_u32(r[a]) : wave_max_u32(r[a]);
+    if (lane_id() == 0)
+        for (int a = 0; a < 7; a++) s_r[a][threadIdx.x >> 6] = r[a];
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    uint32_t status = 0;
+    for (int a = 0; a < 7; a++)
+        for (uint32_t k = 1; k < LPS_BLOCK / 64; k++) r[a] = (a < 3) ? min(r[a], s_r[a][k]) : max(r[a], s_r[a][k]);
+    uint32_t status = r[6] ? (uint32_t)LPS_ERR_NONFINITE : 0u;
     float mn[3], mx[3];
     for (int a = 0; a < 3; a++) {
-        mn[a] = o2f(g->bmin[slot][a]);
-        mx[a] = o2f(g->bmax[slot][a]);
+        mn[a] = o2f(r[a]);
+        mx[a] = o2f(r[3 + a]);
         g->bbox_min[a] = mn[a];
         g->bbox_max[a] = mx[a];
     }
-    if (g->nonfinite[slot]) status = LPS_ERR_NONFINITE;
     const float gs = h * 1.f;
     g->spacing = gs;
     uint64_t cells = 1;
@@ -332,16 +354,13 @@ __global__ void k_grid_setup(DevGrid* g, uint32_t slot, float h, uint32_t cell_c
     g->key_bits = cm1 ? 32u - (uint32_t)__clz(cm1) : 0u;
     g->passes_done = 0u;
     g->status = status;
-    g->done = 0u;
-    g->iterations = 0u;
+    g->stop_iter = 0u;
+    g->best_bits = 0u;
+    g->stall = 0u;
+    g->converged = 0u;
+    g->blocks_done = 0u;
     if (!status) g->steps_ok += 1u;
     for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++) g->err_bits[k] = 0u;
-    const uint32_t o = slot ^ 1u;
-    for (int a = 0; a < 3; a++) {
-        g->bmin[o][a] = f2o(FLT_MAX);
-        g->bmax[o][a] = f2o(FLT_MIN);
-    }
-    g->nonfinite[o] = 0u;
 }
 
 // calcGridIndices (SPHKernels.cl:66-82)
@@ -383,42 +402,40 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_radix_hist(const uint32_t* __rest
     hist[threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (the same bucket-major order
-// as the reference's host scan, SPHSolver.cpp:1069-1078), one block.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist, uint32_t total, DevGrid* g,
-                                                     uint32_t shift) {
+// Exclusive scan of each digit's per-tile counts (one block per digit); the
+// digit totals are scanned by every scatter block.  Together this is the
+// bucket-major scan of the reference (SPHSolver.cpp:1069-1078).
+__global__ __launch_bounds__(LPS_BLOCK) void k_radix_scan(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                          uint32_t* __restrict__ digit_total, DevGrid* g,
+                                                          uint32_t shift) {
     if (step_dead(g) || shift >= g->key_bits) return;
-    __shared__ uint32_t wsum[16];
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (total + 1023u) / 1024u;
-    const uint32_t s = min(t * per, total), e = min(s + per, total);
-    uint32_t sum = 0;
-    for (uint32_t k = s; k < e; k++) sum += hist[k];
-    // block exclusive scan of sum
-    uint32_t incl = sum;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
-        if (lane_id() >= (uint32_t)off) incl += y;
-    }
-    if (lane_id() == 63) wsum[t >> 6] = incl;
-    __syncthreads();
-    if (t < 64) {
-        uint32_t v = t < 16 ? wsum[t] : 0u;
-        uint32_t inc = v;
+    __shared__ uint32_t wsum[LPS_BLOCK / 64];
+    const uint32_t d = blockIdx.x, t = threadIdx.x, lane = lane_id(), w = t >> 6;
+    uint32_t* row = hist + d * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += LPS_BLOCK) {
+        const uint32_t k = base + t;
+        const uint32_t v = k < ntiles ? row[k] : 0u;
+        uint32_t incl = v;
         for (int off = 1; off < 64; off <<= 1) {
-            const uint32_t y = (uint32_t)__shfl_up((int)inc, off, 64);
-            if (lane_id() >= (uint32_t)off) inc += y;
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+            if (lane >= (uint32_t)off) incl += y;
         }
-        if (t < 16) wsum[t] = inc - v;
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        uint32_t wpre = 0, tot = 0;
+        for (uint32_t q = 0; q < LPS_BLOCK / 64; q++) {
+            if (q < w) wpre += wsum[q];
+            tot += wsum[q];
+        }
+        if (k < ntiles) row[k] = carry + wpre + incl - v;
+        carry += tot;
+        __syncthreads();
     }
-    __syncthreads();
-    uint32_t off = wsum[t >> 6] + incl - sum;
-    for (uint32_t k = s; k < e; k++) {
-        const uint32_t v = hist[k];
-        hist[k] = off;
-        off += v;
+    if (t == 0) {
+        digit_total[d] = carry;
+        if (d == 0) g->passes_done += 1u;
     }
-    if (t == 0) g->passes_done += 1u;
 }
 
 // Stable scatter of one tile: items are taken in index order (round-major,
@@ -428,13 +445,28 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_radix_scatter(const uint32_t* __r
                                                              const uint32_t* __restrict__ vin,
                                                              uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                              uint32_t n, uint32_t shift, const DevGrid* g,
-                                                             const uint32_t* __restrict__ hist, uint32_t ntiles) {
+                                                             const uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                             const uint32_t* __restrict__ digit_total) {
     if (step_dead(g) || shift >= g->key_bits) return;
     __shared__ uint32_t s_off[256];
     __shared__ uint32_t s_w[LPS_BLOCK / 64][256];
+    __shared__ uint32_t s_ws[LPS_BLOCK / 64];
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
     const uint32_t tile = blockIdx.x;
-    s_off[t] = hist[t * ntiles + tile];
+    {
+        // exclusive scan of the 256 digit totals
+        const uint32_t v = digit_total[t];
+        uint32_t incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
+            if (lane >= (uint32_t)off) incl += y;
+        }
+        if (lane == 63) s_ws[w] = incl;
+        __syncthreads();
+        uint32_t wpre = 0;
+        for (uint32_t q = 0; q < w; q++) wpre += s_ws[q];
+        s_off[t] = hist[t * ntiles + tile] + wpre + incl - v;
+    }
     for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) s_w[k][t] = 0u;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
@@ -718,7 +750,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
         const float s2 = dx * dx + dy * dy + dz * dz;
         if (s2 <= c.s_max && i != j) {
             const float d = sqrtf(s2);
-            const float ux = (xi.x - xj.x) / d, uy = (xi.y - xj.y) / d, uz = (xi.z - xj.z) / d;
+            float ux, uy, uz;
+            pair_dir(xi, xj, d, i, j, ux, uy, uz);
             const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
             const float sc = tf + q[j];
             pfx += ux * sc * w;
@@ -734,21 +767,38 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
 }
 
 // pciCalcDensityPressure (SPHKernels.cl:737-804) + the density-error maximum
-// of iteration `slot` (max_i |e_i|, e_i = rho*_i - rho0 scaled by npf when the
+// of correction k (max_i |e_i|, e_i = rho*_i - rho0 scaled by npf when the
 // pressure update is negative; PCISPHSolver.cpp:113-121,152-156).  Cells come
 // from the current position, distances from the predicted ones.
+// The last block to finish evaluates the loop rule for correction k (1-based):
+// stop when k >= min and the ratio is below the tolerance (converged), when
+// k >= min and the maximum error has not improved on its best value for
+// `stall_limit` corrections, or at k == max; a NaN error marks the step
+// non-finite.  It writes {state, corrections, error bits, converged} to the
+// host slot (state 0 continue, 1 stopped, 2 error).
 __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
-    float* __restrict__ q, uint32_t first_iter, uint32_t slot, uint32_t nlogical) {
-    if (step_dead(g) || g->done) return;
+    float* __restrict__ q, uint32_t k, LoopRule rule, uint32_t* __restrict__ host_flag, uint32_t nlogical) {
+    const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
+    if (step_dead(g) || g->stop_iter != 0u) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            const uint32_t st = step_dead(g) ? 2u : 1u;
+            const uint32_t si = g->stop_iter;
+            host_flag[0] = st;
+            host_flag[1] = si;
+            host_flag[2] = si ? g->err_bits[min(si - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u)] : 0u;
+            host_flag[3] = g->converged;
+        }
+        return;
+    }
     const uint32_t lb = xcd_block(nlogical);
-    if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
+    __shared__ uint32_t s_last;
     load_table(sW, tables);
     __syncthreads();
-    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    const uint32_t i = (lb == UINT_MAX) ? UINT_MAX : lb * LPS_BLOCK + threadIdx.x;
     uint32_t ebits = 0;
     if (i < c.n) {
         const GridView G = grid_view(g);
@@ -771,7 +821,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
             dp *= c.npf;
             e *= c.npf;
         }
-        const float p = (first_iter ? 0.f : press[i]) + dp;
+        const float p = (k == 1u ? 0.f : press[i]) + dp;
         rho_pred[i] = rp;
         press[i] = p;
         q[i] = p / (rp * rp);
@@ -782,9 +832,44 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t m = 0;
-        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) m = max(m, s_max[k]);
+        for (uint32_t w = 0; w < LPS_BLOCK / 64; w++) m = max(m, s_max[w]);
         if (m) atomicMax(&g->err_bits[slot], m);
+        __threadfence();
+        s_last = (atomicAdd(&g->blocks_done, 1u) == gridDim.x - 1u) ? 1u : 0u;
     }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+    // last block: every block's maximum is visible (fenced before its ticket)
+    __threadfence();
+    g->blocks_done = 0u;
+    const uint32_t eb = atomicOr(&g->err_bits[slot], 0u);
+    uint32_t state = 0;
+    if (eb >= 0x7F800000u) {
+        g->status = LPS_ERR_NONFINITE;
+        state = 2;
+    } else {
+        // non-negative floats order like their bit patterns
+        if (k == 1u || eb < g->best_bits) {
+            g->best_bits = eb;
+            g->stall = 0u;
+        } else {
+            g->stall += 1u;
+        }
+        const float ratio = __uint_as_float(eb) / c.rest_density;
+        if (k >= rule.min_iter && ratio < rule.tol) {
+            state = 1;
+            g->converged = 1u;
+        } else if (k >= rule.min_iter && rule.stall_limit > 0u && g->stall >= rule.stall_limit) {
+            state = 1;
+        } else if (k >= rule.max_iter) {
+            state = 1;
+        }
+        if (state == 1u) g->stop_iter = k;
+    }
+    host_flag[0] = state;
+    host_flag[1] = k;
+    host_flag[2] = eb;
+    host_flag[3] = g->converged;
 }
 
 // pciCalcPressureForce (SPHKernels.cl:806-878) followed by the next
@@ -795,8 +880,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres,
-    const lps_plane* __restrict__ planes, float4* __restrict__ xpred_next, uint32_t nlogical) {
-    if (step_dead(g) || g->done) return;
+    const lps_plane* __restrict__ planes, float4* __restrict__ xpred_next, uint32_t k, uint32_t nlogical) {
+    // runs for correction k unless the loop stopped at an earlier correction
+    if (step_dead(g) || (g->stop_iter != 0u && g->stop_iter < k)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sP1[LPS_TABLE_SIZE];
@@ -815,7 +901,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
         const float s2 = dx * dx + dy * dy + dz * dz;
         if (s2 <= c.s_max && i != j) {
             const float d = sqrtf(s2);
-            const float ux = (xi.x - xj.x) / d, uy = (xi.y - xj.y) / d, uz = (xi.z - xj.z) / d;
+            float ux, uy, uz;
+            pair_dir(xi, xj, d, i, j, ux, uy, uz);
             const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
             const float sc = tf + q[j];
             pfx += ux * sc * w;
@@ -832,35 +919,6 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
                             f.y + f4.y, f.z + f4.z, xr.w, c, boxes, spheres, planes);
 }
 
-// Convergence of iteration k (1-based): stop when k >= min and the ratio is
-// below the tolerance; a NaN error marks the step non-finite.
-__global__ void k_pci_check(DevGrid* g, uint32_t k, uint32_t slot, uint32_t min_iter, float rest_density,
-                            float tol, uint32_t* __restrict__ host_flag) {
-    if (threadIdx.x != 0) return;
-    uint32_t state = 0;  // 1: converged / stopped, 2: error
-    if (g->status != 0u) {
-        state = 2;
-    } else if (g->done) {
-        state = 1;
-    } else {
-        const uint32_t eb = g->err_bits[slot];
-        if (eb >= 0x7F800000u) {
-            g->status = LPS_ERR_NONFINITE;
-            state = 2;
-        } else {
-            const float ratio = __uint_as_float(eb) / rest_density;
-            if (k >= min_iter && ratio < tol) state = 1;
-        }
-    }
-    if (state == 1u && !g->done) {
-        g->done = 1u;
-        g->iterations = k;
-    }
-    host_flag[0] = state;
-    host_flag[1] = g->done ? g->iterations : k;
-    host_flag[2] = g->err_bits[slot];
-}
-
 // integrate (SPHKernels.cl:424-455) + pciAddPressureForce (SPHKernels.cl:880-890)
 // + handleCollisions (SPHKernels.cl:457-486); accumulates the bounding box of
 // the new positions for the next step.
@@ -869,7 +927,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     const float4* __restrict__ force, const float4* __restrict__ fp, const float4* __restrict__ posrho,
     float* __restrict__ pprev, const float* __restrict__ press, DevGrid* g, SimConst c,
     const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres, const lps_plane* __restrict__ planes,
-    uint32_t next_slot, uint32_t nlogical) {
+    uint32_t* __restrict__ bbox_part, uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const bool dead = step_dead(g);
@@ -906,7 +964,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
             if (pprev) pprev[i] = press[i];
         }
     }
-    bbox_accumulate(g, next_slot, valid, p.x, p.y, p.z);
+    bbox_partial(bbox_part, lb, valid, p.x, p.y, p.z);
 }
 
 // ====================================================== host launchers ====
@@ -915,15 +973,15 @@ namespace lps_dev {
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 static inline uint32_t xcd_grid(uint32_t nlogical) { return cdiv(nlogical, LPS_XCDS) * LPS_XCDS; }
 
-void launch_bbox_reset(hipStream_t s, DevGrid* g, uint32_t slot) { k_bbox_reset<<<1, 64, 0, s>>>(g, slot); }
-
-void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, DevGrid* g, uint32_t slot) {
+void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, uint32_t* part) {
     const uint32_t nl = cdiv(n, LPS_BLOCK);
-    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, n, g, slot, nl);
+    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, n, part, nl);
 }
 
-void launch_grid_setup(hipStream_t s, DevGrid* g, uint32_t slot, float h, uint32_t cell_cap) {
-    k_grid_setup<<<1, 64, 0, s>>>(g, slot, h, cell_cap);
+uint32_t bbox_parts(uint32_t n) { return cdiv(n, LPS_BLOCK); }
+
+void launch_grid_setup(hipStream_t s, DevGrid* g, const uint32_t* part, uint32_t n, float h, uint32_t cell_cap) {
+    k_grid_setup<<<1, LPS_BLOCK, 0, s>>>(g, part, cdiv(n, LPS_BLOCK), h, cell_cap);
 }
 
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, uint32_t* keys, uint32_t* vals) {
@@ -933,11 +991,11 @@ void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g,
 uint32_t radix_tiles(uint32_t n) { return cdiv(n, LPS_RADIX_TILE); }
 
 void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
-                       uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist) {
+                       uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist, uint32_t* digit_total) {
     const uint32_t nt = radix_tiles(n);
     k_radix_hist<<<nt, LPS_BLOCK, 0, s>>>(kin, n, shift, g, hist, nt);
-    k_radix_scan<<<1, 1024, 0, s>>>(hist, nt * 256u, g, shift);
-    k_radix_scatter<<<nt, LPS_BLOCK, 0, s>>>(kin, vin, kout, vout, n, shift, g, hist, nt);
+    k_radix_scan<<<256, LPS_BLOCK, 0, s>>>(hist, nt, digit_total, g, shift);
+    k_radix_scatter<<<nt, LPS_BLOCK, 0, s>>>(kin, vin, kout, vout, n, shift, g, hist, nt, digit_total);
 }
 
 void launch_reorder(hipStream_t s, const ReorderArgs& a) {
@@ -975,30 +1033,25 @@ void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
 }
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, bool first_iter, uint32_t slot) {
+                        float* q, uint32_t k, const LoopRule& rule, uint32_t* host_flag) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_pci_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q,
-                                                     first_iter ? 1u : 0u, slot, nl);
+                                                     k, rule, host_flag, nl);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                         const float* q, float4* fp, float4* xpred_next) {
+                         const float* q, float4* fp, float4* xpred_next, uint32_t k) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_pci_pressure<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp,
                                                       a.force, a.posrho, a.vel, a.vh, a.first, a.boxes, a.spheres,
-                                                      a.planes, xpred_next, nl);
-}
-
-void launch_pci_check(hipStream_t s, DevGrid* g, uint32_t k, uint32_t slot, uint32_t min_iter, float rest_density,
-                      float tol, uint32_t* host_flag) {
-    k_pci_check<<<1, 64, 0, s>>>(g, k, slot, min_iter, rest_density, tol, host_flag);
+                                                      a.planes, xpred_next, k, nl);
 }
 
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
-                      uint32_t next_slot) {
+                      uint32_t* bbox_part) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_integrate<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.vel, a.vh, a.first, a.force, fp, a.posrho, pprev, press,
-                                                   a.g, a.c, a.boxes, a.spheres, a.planes, next_slot, nl);
+                                                   a.g, a.c, a.boxes, a.spheres, a.planes, bbox_part, nl);
 }
 
 }  // namespace lps_dev

@@ -49,6 +49,28 @@ def jitter(seed: int, count: int, amp: float) -> np.ndarray:
     return np.float32(amp) * (np.float32(2.0) * u - np.float32(1.0))
 
 
+def spawn_cube(position, half_sizes, particle_radius) -> np.ndarray:
+    """SPHParticleEmitter::spawnCube (SPHParticleEmitter.cpp:13-43) in FP32: a
+    lattice of step 1.6 r_p filling the cube, centred by half the remainder,
+    in x-outer / z-inner order."""
+    f = np.float32
+    r = f(particle_radius)
+    step = f(1.6) * r
+    pos = np.asarray(position, np.float32)
+    half = np.asarray(half_sizes, np.float32)
+    size = ((f(2.0) * half) - (f(2.0) * r)) / step
+    res = size.astype(np.int32) + 1
+    rem = size - size.astype(np.int32).astype(np.float32)
+    off = (f(1.0) + rem / f(2.0)) * r
+    base = (pos - half) + off
+    ix = np.arange(res[0]).astype(np.float32) * step
+    iy = np.arange(res[1]).astype(np.float32) * step
+    iz = np.arange(res[2]).astype(np.float32) * step
+    X, Y, Z = np.meshgrid(ix, iy, iz, indexing="ij")
+    lat = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    return (base + lat).astype(np.float32)
+
+
 def block_lattice(nx: int, ny: int, nz: int, s: float, origin=(0.0, 0.0, 0.0)) -> np.ndarray:
     """[n,3] FP32 lattice origin + s*(i+1), x-outer / z-inner order."""
     s32 = np.float32(s)
@@ -96,6 +118,7 @@ def _fluid_params(s: float, method: int, **kw) -> Params:
         min_iterations=3,
         max_iterations=50,
         max_density_error_ratio=0.01,
+        stall_iterations=3,
     )
     return p.replace(**kw) if kw else p
 

@@ -81,6 +81,22 @@ Tables<T> make_tables(T radius) {
     return t;
 }
 
+template <class T>
+Tables<T> cast_tables(const Tables<float>& f) {
+    Tables<T> t;
+    t.h = (T)f.h;
+    t.div = (T)f.div;
+    t.r2 = (T)f.r2;
+    for (size_t i = 0; i < f.w.size(); i++) {
+        t.w.push_back((T)f.w[i]);
+        t.w1.push_back((T)f.w1[i]);
+        t.w2.push_back((T)f.w2[i]);
+        t.p1.push_back((T)f.p1[i]);
+        t.v2.push_back((T)f.v2[i]);
+    }
+    return t;
+}
+
 // SPHKernel::get*Weight (SPHKernel.cpp:13-44): zero at or beyond the radius,
 // otherwise table[int(d / step)].
 template <class T>
@@ -268,8 +284,9 @@ Grid build_grid(const orc_params* p, int32_t n, const PosT* x, bool use_radix) {
     g.skeys.resize(n);
     for (int32_t i = 0; i < n; i++) g.skeys[i] = g.keys[g.perm[i]];
     g.lb.resize((size_t)g.cells + 1);
-    for (uint32_t c = 0; c <= g.cells; c++)
-        g.lb[c] = (uint32_t)(std::lower_bound(g.skeys.begin(), g.skeys.end(), c) - g.skeys.begin());
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c <= (int64_t)g.cells; c++)
+        g.lb[c] = (uint32_t)(std::lower_bound(g.skeys.begin(), g.skeys.end(), (uint32_t)c) - g.skeys.begin());
     return g;
 }
 
@@ -307,6 +324,19 @@ inline T dist(const V3<T>& a, const V3<T>& b) {
 }
 template <class T>
 inline T len3(T x, T y, T z) { return std::sqrt(x * x + y * y + z * z); }
+
+// Pressure-force direction (x_i - x_j) / d of the OpenCL kernels
+// (SPHKernels.cl:406,865).  For two distinct particles at exactly the same
+// point (d == 0) the reference yields 0/0 = NaN (hazard H9; the box collider
+// snaps every particle that crosses three walls onto the same corner point).
+// Here such a pair gets the unit direction +y for the higher sorted index and
+// -y for the lower one: antisymmetric, so momentum is still conserved, and it
+// separates the pair instead of leaving it stacked.
+template <class T>
+inline V3<T> pair_dir(const V3<T>& xi, const V3<T>& xj, T d, int32_t i, int32_t j) {
+    if (d > (T)0) return V3<T>{(xi.x - xj.x) / d, (xi.y - xj.y) / d, (xi.z - xj.z) / d};
+    return V3<T>{(T)0, i > j ? (T)1 : (T)-1, (T)0};
+}
 
 struct Colliders {
     const orc_box* boxes; int32_t nbox;
@@ -391,7 +421,9 @@ inline void collide_sphere_ocl(V3<T>& pos, V3<T>& vel, T r, T e, T mu, const orc
     if (!s.boundary) {
         T pen = (R + r) - d;
         if (pen > 0) {
-            V3<T> n{(pos.x - c.x) / d, (pos.y - c.y) / d, (pos.z - c.z) / d};
+            // a particle exactly at the centre has no normal (0/0 in the
+            // reference); it is pushed along +y
+            V3<T> n = d > 0 ? V3<T>{(pos.x - c.x) / d, (pos.y - c.y) / d, (pos.z - c.z) / d} : V3<T>{0, 1, 0};
             T k = R + r;
             V3<T> pt{c.x + n.x * k, c.y + n.y * k, c.z + n.z * k};
             resolve(pos, vel, e, mu, n, pt);
@@ -507,14 +539,21 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
     if (tr) { tr->status = ST_OK; tr->iterations = 0; }
     if (n <= 0) return ST_INVALID;
     Tables<float> tf = make_tables<float>(kernel_radius<float>(p));
-    Tables<T> tb = make_tables<T>(kernel_radius<T>(p));
-    const T m = particle_mass<T>(p, tb);
+    // The FP64 rerun takes the FP32 model (tables, mass) as exact inputs and
+    // evaluates the step in double: deriving them in double would move the
+    // lattice distances across lookup-index ties (d / step = 500 exactly for
+    // the nearest lattice neighbours), a 0.2 % change of the mass that is a
+    // property of the piecewise-constant tables, not of the step's arithmetic.
+    Tables<T> tb = cast_tables<T>(tf);
+    const T m = (T)particle_mass<float>(p, tf);
     const T dt = (T)dtf;
     const T h = tb.h, ds = tb.div;
     const T rho0 = (T)p->rest_density;
 
-    // ---- grid (from FP32 positions) and reorder (SPHSolver.cpp:1008-1132)
-    Grid g = build_grid(p, n, x, false);
+    // ---- grid (from FP32 positions) and reorder (SPHSolver.cpp:1008-1132);
+    // the multi-threaded baseline sorts with the reference's radix scheme, the
+    // checker with std::stable_sort (same permutation, checked in tests)
+    Grid g = build_grid(p, n, x, nthreads != 1);
     fill_trace_grid(g, n, tr);
     if (g.status != ST_OK) {
         if (tr) tr->status = g.status;
@@ -611,7 +650,7 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
                 V3<T> xj = s.x[j];
                 T d = dist(xj, xi);
                 if (d < h) {
-                    V3<T> dir{(xi.x - xj.x) / d, (xi.y - xj.y) / d, (xi.z - xj.z) / d};
+                    V3<T> dir = pair_dir(xi, xj, d, i, j);
                     uint32_t idx = (uint32_t)std::trunc(d / ds);
                     if (i != j) {
                         T rj = s.rho[j];
@@ -649,6 +688,8 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
         int32_t minit = p->min_iterations;
         int32_t k = 0;
         bool nonfinite = false;
+        T best = 0;
+        int32_t stall = 0, converged = 0;
         for (k = 1; k <= maxit; k++) {
             // pciIntegrate + pciHandleCollisions (SPHKernels.cl:676-735)
 #pragma omp parallel for schedule(static)
@@ -703,7 +744,7 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
                     T d = dist(xpj, xpi);
                     if (d < h) {
                         if (i != j) {
-                            V3<T> dir{(xpi.x - xpj.x) / d, (xpi.y - xpj.y) / d, (xpi.z - xpj.z) / d};
+                            V3<T> dir = pair_dir(xpi, xpj, d, i, j);
                             uint32_t idx = (uint32_t)std::trunc(d / ds);
                             T rj = s.rp[j];
                             T sc = tfac + pp[j] / (rj * rj);
@@ -725,10 +766,25 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
             if (sizeof(T) == sizeof(double)) ratio = (float)(mx / rho0);
             if (tr && k <= 64) tr->err_ratio[k - 1] = ratio;
             if (nonfinite) break;
-            if (k >= minit && ratio < p->max_density_error_ratio) break;
+            // stall rule (no reference counterpart): count corrections that do
+            // not improve on the best maximum error so far
+            if (k == 1 || mx < best) {
+                best = mx;
+                stall = 0;
+            } else {
+                stall++;
+            }
+            if (k >= minit && ratio < p->max_density_error_ratio) {
+                converged = 1;
+                break;
+            }
+            if (k >= minit && p->stall_iterations > 0 && stall >= p->stall_iterations) break;
         }
         if (k > maxit) k = maxit;
-        if (tr) tr->iterations = k;
+        if (tr) {
+            tr->iterations = k;
+            tr->converged = converged;
+        }
         // pciAddPressureForce (SPHKernels.cl:880-890)
         for (int32_t i = 0; i < n; i++) {
             s.f[i].x += s.fp[i].x; s.f[i].y += s.fp[i].y; s.f[i].z += s.fp[i].z;

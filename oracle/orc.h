@@ -62,6 +62,7 @@ typedef struct orc_params {
     int32_t max_iterations;
     float max_density_error_ratio;
     int32_t warm_start;
+    int32_t stall_iterations;
 } orc_params;
 
 typedef struct orc_box { float center[3]; float half[3]; int32_t boundary; } orc_box;
@@ -90,6 +91,7 @@ typedef struct orc_trace {
     int32_t iterations;
     float delta;
     float err_ratio[64];   /* max density error ratio per iteration (first 64) */
+    int32_t converged;     /* PCISPH: tolerance met */
     int32_t status;
 } orc_trace;
 

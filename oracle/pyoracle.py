@@ -46,6 +46,7 @@ class COrcTrace(C.Structure):
         ("iterations", C.c_int32),
         ("delta", C.c_float),
         ("err_ratio", C.c_float * 64),
+        ("converged", C.c_int32),
         ("status", C.c_int32),
     ]
 
@@ -180,6 +181,7 @@ class Trace:
     iterations: int = 0
     delta: float = 0.0
     err_ratio: Optional[np.ndarray] = None
+    converged: int = 0
 
 
 def _mk_trace(n, cell_cap, fdtype):
@@ -204,7 +206,7 @@ def _read_trace(tr, bufs, n) -> Trace:
               bbox_max=np.array(tr.bbox_max[:], np.float32),
               grid_offset=np.array(tr.grid_offset[:], np.float32), grid_spacing=tr.grid_spacing,
               grid_size=np.array(tr.grid_size[:], np.uint32), cell_count=tr.cell_count,
-              iterations=tr.iterations, delta=tr.delta,
+              iterations=tr.iterations, delta=tr.delta, converged=tr.converged,
               err_ratio=np.array(tr.err_ratio[:max(0, min(tr.iterations, 64))], np.float32))
     t.keys = bufs["keys"]
     t.perm = bufs["perm"]
