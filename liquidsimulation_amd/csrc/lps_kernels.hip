@@ -103,6 +103,56 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
     }
 }
 
+// The same stencil in two passes per window of up to 32 candidates of a row:
+// pass 1 tests d^2 <= s_max for every candidate (four independent loads in
+// flight) and records a per-lane accept mask; pass 2 visits the accepted
+// candidates in ascending order.  The accepted set and the visiting order are
+// those of for_neighbors, so sums are bit-identical, but the expensive pair
+// work (IEEE sqrt and divisions) no longer runs once per candidate for the
+// whole wave, only once per accepted pair of the busiest lane.
+template <class Acc, class F>
+__device__ __forceinline__ void for_neighbors_masked(const GridView& G, const uint32_t* __restrict__ lb, float ax,
+                                                     float ay, float az, const float4* __restrict__ xs, float px,
+                                                     float py, float pz, float s_max, Acc&& accept_extra, F&& fn) {
+    const int xg = (int)truncf((ax + G.off0) / G.gs);
+    const int yg = (int)truncf((ay + G.off1) / G.gs);
+    const int zg = (int)truncf((az + G.off2) / G.gs);
+    const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
+    auto hit = [&](const float4& q) {
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        const float s2 = dx * dx + dy * dy + dz * dz;
+        return s2 <= s_max;
+    };
+    for (int z = zg - 1; z <= zg + 1; z++) {
+        if (z < 0 || z >= G.gz) continue;
+        for (int y = yg - 1; y <= yg + 1; y++) {
+            if (y < 0 || y >= G.gy) continue;
+            const uint32_t base = (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+            const uint32_t b = lb[base + (uint32_t)x0];
+            const uint32_t e = lb[base + (uint32_t)x1 + 1u];
+            for (uint32_t w0 = b; w0 < e; w0 += 32u) {
+                const uint32_t n = min(32u, e - w0);
+                uint32_t mask = 0u;
+                uint32_t u = 0;
+                for (; u + 4u <= n; u += 4u) {
+                    const float4 q0 = xs[w0 + u], q1 = xs[w0 + u + 1u], q2 = xs[w0 + u + 2u], q3 = xs[w0 + u + 3u];
+                    mask |= (hit(q0) ? 1u : 0u) << u;
+                    mask |= (hit(q1) ? 2u : 0u) << u;
+                    mask |= (hit(q2) ? 4u : 0u) << u;
+                    mask |= (hit(q3) ? 8u : 0u) << u;
+                }
+                for (; u < n; u++) mask |= (hit(xs[w0 + u]) ? 1u : 0u) << u;
+                while (mask) {
+                    const uint32_t k = (uint32_t)__ffs((int)mask) - 1u;
+                    mask &= mask - 1u;
+                    const uint32_t j = w0 + k;
+                    if (accept_extra(j)) fn(j);
+                }
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------- colliders --
 // resolveCollision (SPHKernels.cl:640-659)
 __device__ __forceinline__ void resolve(float3& pos, float3& vel, float e, float mu, float3 n, float3 pt) {
@@ -644,14 +694,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     const GridView G = grid_view(g);
     const float4 xi = pos[i];
     float sum = 0.f;
-    for_neighbors(G, cell_lb, xi.x, xi.y, xi.z, [&](uint32_t j) {
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max,
+                         [](uint32_t) { return true; }, [&](uint32_t j) {
         const float4 xj = pos[j];
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float s2 = dx * dx + dy * dy + dz * dz;
-        if (s2 <= c.s_max) {
-            const float d = sqrtf(s2);
-            sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-        }
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
     });
     const float rho = c.mass * sum;
     float p = c.stiffness * (rho - c.rest_density);
@@ -686,26 +734,24 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     const float ri = xi.w;
     float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
-    for_neighbors(G, cell_lb, xi.x, xi.y, xi.z, [&](uint32_t j) {
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max,
+                         [](uint32_t) { return true; }, [&](uint32_t j) {
         const float4 xj = posrho[j];
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float s2 = dx * dx + dy * dy + dz * dz;
-        if (s2 <= c.s_max) {
-            const float d = sqrtf(s2);
-            const uint32_t idx = min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1));
-            const float rj = xj.w;
-            const float w1 = sW1[idx];
-            snx += (xi.x - xj.x) * w1 / rj;
-            sny += (xi.y - xj.y) * w1 / rj;
-            snz += (xi.z - xj.z) * w1 / rj;
-            lc += sW2[idx] / rj;
-            if (i != j) {
-                const float4 vj = vel[j];
-                const float wv = sV2[idx];
-                vfx += (vj.x - vi.x) * wv / rj;
-                vfy += (vj.y - vi.y) * wv / rj;
-                vfz += (vj.z - vi.z) * wv / rj;
-            }
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        const uint32_t idx = min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1));
+        const float rj = xj.w;
+        const float w1 = sW1[idx];
+        snx += (xi.x - xj.x) * w1 / rj;
+        sny += (xi.y - xj.y) * w1 / rj;
+        snz += (xi.z - xj.z) * w1 / rj;
+        lc += sW2[idx] / rj;
+        if (i != j) {
+            const float4 vj = vel[j];
+            const float wv = sV2[idx];
+            vfx += (vj.x - vi.x) * wv / rj;
+            vfy += (vj.y - vi.y) * wv / rj;
+            vfz += (vj.z - vi.z) * wv / rj;
         }
     });
     snx *= c.mass; sny *= c.mass; snz *= c.mass;
@@ -744,12 +790,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const float4 xi = posrho[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors(G, cell_lb, xi.x, xi.y, xi.z, [&](uint32_t j) {
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max,
+                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) {
         const float4 xj = posrho[j];
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float s2 = dx * dx + dy * dy + dz * dz;
-        if (s2 <= c.s_max && i != j) {
-            const float d = sqrtf(s2);
+        {
+            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
             float ux, uy, uz;
             pair_dir(xi, xj, d, i, j, ux, uy, uz);
             const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
@@ -805,14 +851,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         const float4 xa = pos[i];
         const float4 xi = xpred[i];
         float sum = 0.f;
-        for_neighbors(G, cell_lb, xa.x, xa.y, xa.z, [&](uint32_t j) {
+        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max,
+                             [](uint32_t) { return true; }, [&](uint32_t j) {
             const float4 xj = xpred[j];
             const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            const float s2 = dx * dx + dy * dy + dz * dz;
-            if (s2 <= c.s_max) {
-                const float d = sqrtf(s2);
-                sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-            }
+            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+            sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
         });
         const float rp = c.mass * sum;
         float e = rp - c.rest_density;
@@ -895,20 +939,18 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4 xi = xpred[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors(G, cell_lb, xa.x, xa.y, xa.z, [&](uint32_t j) {
+    for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max,
+                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) {
         const float4 xj = xpred[j];
         const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float s2 = dx * dx + dy * dy + dz * dz;
-        if (s2 <= c.s_max && i != j) {
-            const float d = sqrtf(s2);
-            float ux, uy, uz;
-            pair_dir(xi, xj, d, i, j, ux, uy, uz);
-            const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-            const float sc = tf + q[j];
-            pfx += ux * sc * w;
-            pfy += uy * sc * w;
-            pfz += uz * sc * w;
-        }
+        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+        float ux, uy, uz;
+        pair_dir(xi, xj, d, i, j, ux, uy, uz);
+        const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
+        const float sc = tf + q[j];
+        pfx += ux * sc * w;
+        pfy += uy * sc * w;
+        pfz += uz * sc * w;
     });
     const float ri = rho_pred[i];
     const float4 f4 = make_float4((-pfx) * c.mass * ri, (-pfy) * c.mass * ri, (-pfz) * c.mass * ri, 0.f);
