@@ -141,15 +141,68 @@ int lps_get_grid(lps_ctx* ctx, lps_grid_info* info, uint32_t* keys, uint32_t* pe
 int lps_get_fields(lps_ctx* ctx, float* rho, float* pressure, float* force /* [3n] */,
                    float* rho_pred);
 
-/* Learned warm start (PCISPH): 16 -> 64 -> 64 -> 1 MLP, BF16 weights. */
+/* Learned warm start (PCISPH only; no counterpart in the reference, whose
+ * pciInit starts every step from p = 0, SPHKernels.cl:661-672).
+ * A per-particle MLP 16 -> 64 -> 64 -> 1 (ReLU), BF16 weights/activations on
+ * the matrix cores with FP32 accumulation, predicts each particle's converged
+ * pressure from features of the step's first correction; the prediction
+ * replaces the first correction's pressure and the unchanged loop runs on to
+ * the same tolerance.  Features (per particle, sorted order):
+ *   0 rho/rho0 - 1   1 neighbours/32   2 velocity divergence*m*dt
+ *   3-5 colour-field normal*h   6 its length*h   7 previous step's pressure/P
+ *   8 |v| dt/h   9 first-step flag   10 |first prediction - x|/h
+ *   11 e1 = rho*_1/rho0 - 1   12 max(e1, 0)   13 first pressure update/P
+ *   14 (rho*_1 - rho)/rho0   15 100 e1^2
+ * with P = delta * rho0 (PCISPH's delta of the step), so features and target
+ * do not depend on particle size or time step.
+ * Parameters (fp32 master copy): W1[64][16], b1[64], W2[64][64], b2[64],
+ * W3[64], b3.  Inputs are normalised as (f - mean) * inv_std; the seed is the
+ * output times target_scale times P.  With warm_start = LPS_WARM_LEARNED and weights
+ * loaded the seed is used unless the running mean of corrections per step
+ * with it exceeds the one without it (every 32nd step runs the other mode to
+ * keep both current); a seeded step that turns non-finite is re-run unseeded. */
 #define LPS_MLP_IN 16
 #define LPS_MLP_HIDDEN 64
 #define LPS_MLP_PARAMS (LPS_MLP_IN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + 1)
 int lps_mlp_set_weights(lps_ctx* ctx, const float* params /* LPS_MLP_PARAMS, fp32 master copy */,
                         const float* feature_scale /* [2*LPS_MLP_IN]: mean, inv_std */, float target_scale);
 int lps_mlp_get_weights(lps_ctx* ctx, float* params, float* feature_scale, float* target_scale);
-/* Dump (features -> converged pressure) pairs of the last PCISPH step. */
-int lps_dump_pairs(lps_ctx* ctx, float* features /* [n*16] */, float* target /* [n] */, uint32_t n);
+/* Compute the features on every PCISPH step (for lps_dump_pairs) when on. */
+int lps_mlp_capture(lps_ctx* ctx, int32_t on);
+/* Features, final pressure and (if the step was seeded) the seed of the last
+ * PCISPH step, sorted order, pressures in Pa.  Any pointer may be NULL. */
+int lps_dump_pairs(lps_ctx* ctx, float* features /* [n*16] */, float* target /* [n] */,
+                   float* seed /* [n] */, uint32_t n);
+/* Device-resident training set: while capacity > 0 and stride > 0, every
+ * PCISPH step that ends without error appends (features, final pressure / P)
+ * of every stride-th particle (offset rotating per step) until capacity rows
+ * are held.  A call with the current capacity keeps the rows and only sets
+ * the stride (0 pauses); another capacity starts an empty set; 0 frees it. */
+int lps_mlp_collect(lps_ctx* ctx, uint64_t capacity, uint32_t stride);
+int lps_mlp_collected(lps_ctx* ctx, uint64_t* rows);
+typedef struct lps_mlp_train_cfg {
+    uint32_t epochs;
+    uint32_t batch;        /* rows per Adam step */
+    float lr, lr_final;    /* cosine decay from lr to lr_final over all steps */
+    float beta1, beta2, eps;
+    int32_t reinit;        /* 1: fresh He-normal weights; 0: continue from the loaded ones */
+    int32_t refit_scale;   /* 1: feature normalisation and target scale from the rows */
+    float over_weight;     /* loss weight of over-predictions (1: plain MSE) */
+    uint64_t seed;         /* initialisation and per-epoch row permutation */
+} lps_mlp_train_cfg;
+/* Train on the collected rows on the device (loss mean(w d^2) with
+ * d = y - target/target_scale, w = over_weight where d > 0, else 1; Adam);
+ * epoch_loss [epochs] may be NULL.  The trained weights become the loaded
+ * ones.  With negative pressures disabled the correction loop never lowers a
+ * pressure, so an over-predicted seed is never corrected back; weighting
+ * over-predictions more keeps the seed below the converged pressure. */
+int lps_mlp_train(lps_ctx* ctx, const lps_mlp_train_cfg* cfg, float* epoch_loss);
+/* Batch inference and gradient with the loaded weights (training checks):
+ * y [n] is the raw network output (before target_scale and P); grad
+ * [LPS_MLP_PARAMS] and loss are those of the training loss above. */
+int lps_mlp_forward(lps_ctx* ctx, const float* x /* [n*16] */, uint32_t n, float* y);
+int lps_mlp_grad(lps_ctx* ctx, const float* x /* [n*16] */, const float* t /* [n] */, uint32_t n,
+                 float over_weight, float* grad, float* loss);
 
 /* Kernel timing (HIP events on the context stream), for measurement only. */
 int lps_profile_enable(lps_ctx* ctx, int32_t on);

@@ -8,7 +8,7 @@ import re
 
 import numpy as np
 
-from .abi import CGridInfo, CParams, CStepStats, CBox, CSphere, CPlane
+from .abi import CGridInfo, CParams, CStepStats, CBox, CSphere, CPlane, CTrainCfg
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblps.so")
@@ -56,7 +56,13 @@ def lib():
     L.lps_get_fields.argtypes = [vp, vp, vp, vp, vp]
     L.lps_mlp_set_weights.argtypes = [vp, _f32p, _f32p, C.c_float]
     L.lps_mlp_get_weights.argtypes = [vp, _f32p, _f32p, C.POINTER(C.c_float)]
-    L.lps_dump_pairs.argtypes = [vp, _f32p, _f32p, C.c_uint32]
+    L.lps_mlp_capture.argtypes = [vp, C.c_int32]
+    L.lps_dump_pairs.argtypes = [vp, vp, vp, vp, C.c_uint32]
+    L.lps_mlp_collect.argtypes = [vp, C.c_uint64, C.c_uint32]
+    L.lps_mlp_collected.argtypes = [vp, C.POINTER(C.c_uint64)]
+    L.lps_mlp_train.argtypes = [vp, C.POINTER(CTrainCfg), vp]
+    L.lps_mlp_forward.argtypes = [vp, _f32p, C.c_uint32, _f32p]
+    L.lps_mlp_grad.argtypes = [vp, _f32p, _f32p, C.c_uint32, C.c_float, _f32p, C.POINTER(C.c_float)]
     L.lps_profile_enable.argtypes = [vp, C.c_int32]
     L.lps_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]
     L.lps_profile_reset.argtypes = [vp]

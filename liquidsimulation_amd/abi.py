@@ -79,6 +79,23 @@ class CStepStats(C.Structure):
     ]
 
 
+class CTrainCfg(C.Structure):
+    """lps_mlp_train_cfg."""
+    _fields_ = [
+        ("epochs", C.c_uint32),
+        ("batch", C.c_uint32),
+        ("lr", C.c_float),
+        ("lr_final", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("eps", C.c_float),
+        ("reinit", C.c_int32),
+        ("refit_scale", C.c_int32),
+        ("over_weight", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+
 class CGridInfo(C.Structure):
     _fields_ = [
         ("bbox_min", C.c_float * 3),

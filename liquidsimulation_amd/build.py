@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblps.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["lps_api.hip", "lps_kernels.hip", "lps_host.cpp"]
+SOURCES = ["lps_api.hip", "lps_kernels.hip", "lps_mlp.hip", "lps_host.cpp"]
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -26,6 +26,7 @@ FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
+    "-Wl,-z,defs",
     "-ffp-contract=off",
     "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <map>
 #include <new>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -78,6 +79,22 @@ struct lps_ctx {
     std::vector<float> mlp_fscale;
     float mlp_tscale = 1.f;
     bool mlp_loaded = false;
+    bool capture = false;         // compute features every PCISPH step (for lps_dump_pairs)
+    float* d_mlp = nullptr;       // params [LPS_MLP_PARAMS] then fscale [2 * LPS_MLP_IN]
+    float* feat = nullptr;        // [cap_n][LPS_MLP_IN] features of the current step
+    float* seed = nullptr;        // [cap_n] seeded pressure of the current step
+    bool feat_valid = false;      // the last step wrote features (and final pressure)
+    bool seed_valid = false;      // the last step applied the seed
+    // fallback: exponential averages of corrections per step with and without the seed
+    double ema_seeded = -1.0, ema_plain = -1.0;
+    bool ws_on = true;
+    uint64_t ws_steps = 0;
+    // training-set collection and training
+    float *coll_x = nullptr, *coll_t = nullptr;
+    uint64_t coll_cap = 0, coll_rows = 0;
+    uint32_t coll_stride = 1, coll_offset = 0;
+    float *d_grad = nullptr, *d_m1 = nullptr, *d_m2 = nullptr, *d_loss = nullptr;
+    double* d_stats = nullptr;
 
     // profiling
     bool prof = false;
@@ -273,6 +290,8 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if ((st = dalloc(&c->press, n))) return st;
     if ((st = dalloc(&c->q, n))) return st;
     if ((st = dalloc(&c->rho_pred, n))) return st;
+    dfree(c->feat);
+    dfree(c->seed);
     c->cap_n = n;
     return LPS_OK;
 }
@@ -287,6 +306,8 @@ void free_all(lps_ctx* c) {
     dfree(c->posrho); dfree(c->force); dfree(c->fp); dfree(c->press); dfree(c->q); dfree(c->rho_pred);
     dfree(c->run_end); dfree(c->cell_lb); dfree(c->block_max);
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
+    dfree(c->d_mlp); dfree(c->feat); dfree(c->seed); dfree(c->coll_x); dfree(c->coll_t);
+    dfree(c->d_grad); dfree(c->d_m1); dfree(c->d_m2); dfree(c->d_loss); dfree(c->d_stats);
     if (c->flags) (void)hipHostFree(c->flags);
     if (c->h_grid) (void)hipHostFree(c->h_grid);
     c->flags = nullptr;
@@ -359,9 +380,50 @@ struct StepResult {
     int32_t iterations = 0;
     uint32_t err_bits = 0;
     int32_t converged = 1;
+    bool ok = true;  // the loop stopped without an error
 };
 
-StepResult enqueue_step(lps_ctx* c, float dt) {
+// ---- learned warm start: when to seed, fallback bookkeeping
+constexpr uint64_t WS_PROBE_PERIOD = 32;  // every 32nd step runs the other mode
+
+bool warm_ready(const lps_ctx* c) {
+    return c->params.method == LPS_METHOD_PCISPH && c->params.warm_start == LPS_WARM_LEARNED && c->mlp_loaded;
+}
+
+// Seed unless the averaged corrections per step with the seed exceed those
+// without it; every WS_PROBE_PERIOD-th step runs the mode not in use so both
+// averages stay current.
+bool seed_this_step(const lps_ctx* c) {
+    if (!warm_ready(c)) return false;
+    if (c->ema_seeded < 0.0) return true;   // first measurement with the seed,
+    if (c->ema_plain < 0.0) return false;   // then one without
+    const bool probe = (c->ws_steps % WS_PROBE_PERIOD) == WS_PROBE_PERIOD - 1u;
+    return probe ? !c->ws_on : c->ws_on;
+}
+
+void ws_record(lps_ctx* c, bool seeded, double iterations) {
+    double& e = seeded ? c->ema_seeded : c->ema_plain;
+    e = e < 0.0 ? iterations : 0.7 * e + 0.3 * iterations;
+    c->ws_on = c->ema_plain < 0.0 || c->ema_seeded < 0.0 || c->ema_seeded <= c->ema_plain;
+    c->ws_steps++;
+}
+
+bool capture_features(const lps_ctx* c, bool seeded) {
+    return c->params.method == LPS_METHOD_PCISPH && (seeded || (c->coll_cap > 0 && c->coll_stride > 0) || c->capture);
+}
+
+int ensure_feat(lps_ctx* c) {
+    int st;
+    if (!c->feat && (st = dalloc(&c->feat, (size_t)c->cap_n * LPS_MLP_IN))) return st;
+    if (!c->seed && (st = dalloc(&c->seed, (size_t)c->cap_n))) return st;
+    return LPS_OK;
+}
+
+lps_dev::MlpArgs mlp_args(const lps_ctx* c) {
+    return lps_dev::MlpArgs{c->d_mlp, c->d_mlp + LPS_MLP_PARAMS, c->mlp_tscale};
+}
+
+StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     StepResult res;
     const hipStream_t s = c->stream;
     const bool pci = c->params.method == LPS_METHOD_PCISPH;
@@ -376,7 +438,7 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
     if (!pci) {
         {
             Timed t(c, "forces");
-            lps_dev::launch_forces(s, a, nullptr);
+            lps_dev::launch_forces(s, a, nullptr, nullptr, nullptr);
         }
         {
             Timed t(c, "pressure_sph");
@@ -387,11 +449,13 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
             lps_dev::launch_integrate(s, a, nullptr, c->pprev[c->cur], c->press, c->bbox_part);
         }
         c->last_fp_valid = 0;
+        c->feat_valid = c->seed_valid = false;
         return res;
     }
+    float* feat = feat_on ? c->feat : nullptr;
     {
         Timed t(c, "forces");
-        lps_dev::launch_forces(s, a, c->xpred[0]);
+        lps_dev::launch_forces(s, a, c->xpred[0], c->pprev[c->cur], feat);
     }
     const uint32_t kmin = (uint32_t)std::max(1, c->params.min_iterations);
     const uint32_t kmax = (uint32_t)std::max(c->params.max_iterations, (int32_t)kmin);
@@ -400,10 +464,17 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
     auto enqueue_iter = [&](uint32_t kk) {
         {
             Timed t(c, "pci_density");
-            lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule,
+            lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule, feat,
                                         c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
         }
         (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
+        if (kk == 1u && seeded) {
+            // the first correction's pressure update is replaced by the seed
+            Timed t(c, "mlp_seed");
+            lps_dev::launch_mlp_seed(s, mlp_args(c), c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
+                                     delta * c->params.rest_density,
+                                     c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid);
+        }
         {
             Timed t(c, "pci_pressure");
             lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp,
@@ -424,15 +495,28 @@ StepResult enqueue_step(lps_ctx* c, float dt) {
         res.iterations = (int32_t)f[1];
         res.err_bits = f[2];
         res.converged = (int32_t)f[3];
+        res.ok = state == 1u;
         if (state != 0u) break;
         k_known = kw;
         if (kw >= kmax) break;
+    }
+    if (feat_on && res.ok && c->coll_stride > 0 && c->coll_rows < c->coll_cap) {
+        // training rows: every stride-th particle's features and final pressure
+        const uint32_t n = c->n, off = c->coll_offset % c->coll_stride;
+        uint64_t count = n > off ? ((uint64_t)(n - off) + c->coll_stride - 1u) / c->coll_stride : 0u;
+        count = std::min<uint64_t>(count, c->coll_cap - c->coll_rows);
+        lps_dev::launch_collect(s, c->feat, c->press, delta * c->params.rest_density, n, c->coll_stride, off,
+                                (uint32_t)count, c->coll_x, c->coll_t, c->coll_rows);
+        c->coll_rows += count;
+        c->coll_offset = (off + 1u) % c->coll_stride;
     }
     {
         Timed t(c, "integrate");
         lps_dev::launch_integrate(s, a, c->fp, c->pprev[c->cur], c->press, c->bbox_part);
     }
     c->last_fp_valid = 1;
+    c->feat_valid = feat_on;
+    c->seed_valid = seeded;
     return res;
 }
 
@@ -633,15 +717,24 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
     const uint32_t base = c->h_grid->steps_ok;
     uint32_t completed = 0;
     int retries = 0;
+    bool retry_plain = false;
     while (completed < nsteps) {
         const uint32_t batch = pci ? 1u : nsteps - completed;
+        const bool seeded = pci && !retry_plain && seed_this_step(c);
+        const bool feat_on = capture_features(c, seeded);
+        if (feat_on) {
+            int st = ensure_feat(c);
+            if (st) return st;
+        }
+        StepResult last;
         for (uint32_t b = 0; b < batch; b++) {
-            StepResult r = enqueue_step(c, dt);
+            StepResult r = enqueue_step(c, dt, seeded, feat_on);
+            last = r;
             if (stats) {
                 lps_step_stats& o = stats[completed + b];
                 o.iterations = r.iterations;
                 o.max_density_error_ratio = __builtin_bit_cast(float, r.err_bits) / c->params.rest_density;
-                o.warm_start_used = 0;
+                o.warm_start_used = seeded ? 1 : 0;
                 o.status = LPS_OK;
                 o.converged = r.converged;
             }
@@ -659,10 +752,21 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
             if (st) return st;
             continue;
         }
+        if (status == LPS_ERR_NONFINITE && seeded && retries < 8) {
+            // the seeded step failed and left the state as it was: run it
+            // again without the seed and count the failure against the seed
+            ++retries;
+            c->ema_seeded = (double)c->params.max_iterations + 1.0;
+            c->ws_on = false;
+            retry_plain = true;
+            continue;
+        }
+        retry_plain = false;
         if (status != 0u) {
             if (stats && completed < nsteps) stats[completed].status = (int32_t)status;
             return (int)status;
         }
+        if (warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
     }
     if (c->prof) prof_collect(c);
     return LPS_OK;
@@ -731,11 +835,24 @@ int lps_get_fields(lps_ctx* c, float* rho, float* pressure, float* force, float*
 }
 
 int lps_mlp_set_weights(lps_ctx* c, const float* params, const float* feature_scale, float target_scale) {
-    if (!c || !params || !feature_scale || !(target_scale > 0.f)) return LPS_ERR_INVALID;
+    if (!c || !params || !feature_scale || !(target_scale > 0.f) || !std::isfinite(target_scale))
+        return LPS_ERR_INVALID;
+    for (uint32_t k = 0; k < LPS_MLP_PARAMS; k++)
+        if (!std::isfinite(params[k])) return LPS_ERR_INVALID;
+    for (uint32_t k = 0; k < 2 * LPS_MLP_IN; k++)
+        if (!std::isfinite(feature_scale[k])) return LPS_ERR_INVALID;
+    if (!c->d_mlp) HIPC(hipMalloc((void**)&c->d_mlp, sizeof(float) * (LPS_MLP_PARAMS + 2 * LPS_MLP_IN)));
     c->mlp_w.assign(params, params + LPS_MLP_PARAMS);
     c->mlp_fscale.assign(feature_scale, feature_scale + 2 * LPS_MLP_IN);
     c->mlp_tscale = target_scale;
+    HIPC(hipMemcpyAsync(c->d_mlp, c->mlp_w.data(), sizeof(float) * LPS_MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->d_mlp + LPS_MLP_PARAMS, c->mlp_fscale.data(), sizeof(float) * 2 * LPS_MLP_IN,
+                        hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
     c->mlp_loaded = true;
+    c->ema_seeded = c->ema_plain = -1.0;
+    c->ws_on = true;
+    c->ws_steps = 0;
     return LPS_OK;
 }
 
@@ -748,10 +865,221 @@ int lps_mlp_get_weights(lps_ctx* c, float* params, float* feature_scale, float* 
     return LPS_OK;
 }
 
-int lps_dump_pairs(lps_ctx* c, float* features, float* target, uint32_t n) {
-    if (!c || !features || !target) return LPS_ERR_INVALID;
+int lps_mlp_capture(lps_ctx* c, int32_t on) {
+    if (!c) return LPS_ERR_INVALID;
+    c->capture = on != 0;
+    return LPS_OK;
+}
+
+int lps_dump_pairs(lps_ctx* c, float* features, float* target, float* seed, uint32_t n) {
+    if (!c || (!features && !target && !seed)) return LPS_ERR_INVALID;
     if (n != c->n) return LPS_ERR_INVALID;
-    return LPS_ERR_STATE;
+    if (!c->have_step || !c->feat_valid || (seed && !c->seed_valid)) return LPS_ERR_STATE;
+    HIPC(hipStreamSynchronize(c->stream));
+    if (features) HIPC(hipMemcpy(features, c->feat, sizeof(float) * LPS_MLP_IN * (size_t)n, hipMemcpyDeviceToHost));
+    if (target) HIPC(hipMemcpy(target, c->press, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    if (seed) HIPC(hipMemcpy(seed, c->seed, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return LPS_OK;
+}
+
+int lps_mlp_collect(lps_ctx* c, uint64_t capacity, uint32_t stride) {
+    if (!c || capacity > (1ull << 32)) return LPS_ERR_INVALID;
+    if (capacity && capacity == c->coll_cap) {
+        c->coll_stride = stride;  // rows kept; 0 pauses
+        return LPS_OK;
+    }
+    HIPC(hipStreamSynchronize(c->stream));
+    dfree(c->coll_x);
+    dfree(c->coll_t);
+    c->coll_cap = c->coll_rows = 0;
+    c->coll_offset = 0;
+    if (!capacity) return LPS_OK;
+    int st;
+    if ((st = dalloc(&c->coll_x, (size_t)capacity * LPS_MLP_IN))) return st;
+    if ((st = dalloc(&c->coll_t, (size_t)capacity))) return st;
+    c->coll_cap = capacity;
+    c->coll_stride = stride;
+    return LPS_OK;
+}
+
+int lps_mlp_collected(lps_ctx* c, uint64_t* rows) {
+    if (!c || !rows) return LPS_ERR_INVALID;
+    *rows = c->coll_rows;
+    return LPS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int ensure_train_buffers(lps_ctx* c) {
+    int st;
+    if (!c->d_mlp && (st = dalloc(&c->d_mlp, LPS_MLP_PARAMS + 2 * LPS_MLP_IN))) return st;
+    if (!c->d_grad && (st = dalloc(&c->d_grad, LPS_MLP_PARAMS))) return st;
+    if (!c->d_m1 && (st = dalloc(&c->d_m1, LPS_MLP_PARAMS))) return st;
+    if (!c->d_m2 && (st = dalloc(&c->d_m2, LPS_MLP_PARAMS))) return st;
+    if (!c->d_loss && (st = dalloc(&c->d_loss, 1))) return st;
+    if (!c->d_stats && (st = dalloc(&c->d_stats, 4 * LPS_MLP_IN))) return st;
+    return LPS_OK;
+}
+
+// He-normal weights (ReLU layers), zero biases.
+void init_weights(std::vector<float>& w, uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    w.assign(LPS_MLP_PARAMS, 0.f);
+    const int H = LPS_MLP_HIDDEN, I = LPS_MLP_IN;
+    std::normal_distribution<float> n1(0.f, std::sqrt(2.f / I)), n2(0.f, std::sqrt(2.f / H)),
+        n3(0.f, std::sqrt(1.f / H));
+    float* p = w.data();
+    for (int k = 0; k < H * I; k++) *p++ = n1(rng);
+    p += H;
+    for (int k = 0; k < H * H; k++) *p++ = n2(rng);
+    p += H;
+    for (int k = 0; k < H; k++) *p++ = n3(rng);
+}
+
+uint64_t gcd64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lps_mlp_train(lps_ctx* c, const lps_mlp_train_cfg* cfg, float* epoch_loss) {
+    if (!c || !cfg || cfg->epochs == 0 || cfg->batch == 0 || !(cfg->lr > 0.f) || !(cfg->lr_final >= 0.f) ||
+        !(cfg->over_weight > 0.f) || !std::isfinite(cfg->over_weight) ||
+        !(cfg->beta1 >= 0.f && cfg->beta1 < 1.f) || !(cfg->beta2 >= 0.f && cfg->beta2 < 1.f) || !(cfg->eps > 0.f))
+        return LPS_ERR_INVALID;
+    if (c->coll_rows == 0) return LPS_ERR_STATE;
+    int st = ensure_train_buffers(c);
+    if (st) return st;
+    const hipStream_t s = c->stream;
+    const uint64_t N = c->coll_rows;
+    if (cfg->refit_scale || !c->mlp_loaded) {
+        // feature normalisation (mean, 1/std; constant features get weight 0)
+        // and target scale (root mean square of the targets)
+        HIPC(hipMemsetAsync(c->d_stats, 0, sizeof(double) * 4 * LPS_MLP_IN, s));
+        lps_dev::launch_feature_stats(s, c->coll_x, N, LPS_MLP_IN, c->d_stats);
+        lps_dev::launch_feature_stats(s, c->coll_t, N, 1, c->d_stats + 2 * LPS_MLP_IN);
+        double h[4 * LPS_MLP_IN];
+        HIPC(hipMemcpyAsync(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        c->mlp_fscale.assign(2 * LPS_MLP_IN, 0.f);
+        for (int f = 0; f < LPS_MLP_IN; f++) {
+            const double mean = h[f] / (double)N;
+            const double var = std::max(h[LPS_MLP_IN + f] / (double)N - mean * mean, 0.0);
+            c->mlp_fscale[f] = (float)mean;
+            c->mlp_fscale[LPS_MLP_IN + f] = var > 1e-12 * std::max(1.0, mean * mean) ? (float)(1.0 / std::sqrt(var)) : 0.f;
+        }
+        const double rms = std::sqrt(h[2 * LPS_MLP_IN + 1] / (double)N);
+        c->mlp_tscale = rms > 0.0 && std::isfinite(rms) ? (float)rms : 1.f;
+    }
+    if (cfg->reinit || !c->mlp_loaded) init_weights(c->mlp_w, cfg->seed);
+    HIPC(hipMemcpyAsync(c->d_mlp, c->mlp_w.data(), sizeof(float) * LPS_MLP_PARAMS, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(c->d_mlp + LPS_MLP_PARAMS, c->mlp_fscale.data(), sizeof(float) * 2 * LPS_MLP_IN,
+                        hipMemcpyHostToDevice, s));
+    HIPC(hipMemsetAsync(c->d_grad, 0, sizeof(float) * LPS_MLP_PARAMS, s));
+    HIPC(hipMemsetAsync(c->d_m1, 0, sizeof(float) * LPS_MLP_PARAMS, s));
+    HIPC(hipMemsetAsync(c->d_m2, 0, sizeof(float) * LPS_MLP_PARAMS, s));
+    const lps_dev::MlpArgs m = mlp_args(c);
+    const uint64_t per_epoch = (N + cfg->batch - 1u) / cfg->batch;
+    const double total = (double)per_epoch * cfg->epochs;
+    std::mt19937_64 rng(cfg->seed ^ 0x9E3779B97F4A7C15ull);
+    uint32_t t = 0;
+    for (uint32_t e = 0; e < cfg->epochs; e++) {
+        // a fresh affine permutation of the rows each epoch
+        uint64_t mul = N > 1 ? 1u + rng() % (N - 1u) : 1u;
+        while (gcd64(mul, N) != 1u) mul = mul + 1u < N ? mul + 1u : 1u;
+        const uint64_t add = rng() % N;
+        HIPC(hipMemsetAsync(c->d_loss, 0, sizeof(float), s));
+        for (uint64_t b = 0; b < per_epoch; b++) {
+            const uint64_t start = b * cfg->batch;
+            const uint32_t rows = (uint32_t)std::min<uint64_t>(cfg->batch, N - start);
+            const lps_dev::RowMap rm{nullptr, start, mul, add, N};
+            lps_dev::launch_mlp_grad(s, m, c->coll_x, c->coll_t, rm, rows, cfg->over_weight, c->d_grad, c->d_loss);
+            const double u = (double)t / std::max(total - 1.0, 1.0);
+            const float lr = (float)(cfg->lr_final + 0.5 * (cfg->lr - cfg->lr_final) * (1.0 + std::cos(3.14159265358979323846 * u)));
+            ++t;
+            lps_dev::launch_adam(s, c->d_mlp, c->d_grad, c->d_m1, c->d_m2, LPS_MLP_PARAMS, lr, cfg->beta1, cfg->beta2,
+                                 cfg->eps, t);
+        }
+        float loss = 0.f;
+        HIPC(hipMemcpyAsync(&loss, c->d_loss, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        if (epoch_loss) epoch_loss[e] = loss / (float)per_epoch;
+    }
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(c->mlp_w.data(), c->d_mlp, sizeof(float) * LPS_MLP_PARAMS, hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    for (float w : c->mlp_w)
+        if (!std::isfinite(w)) {
+            c->mlp_loaded = false;
+            return LPS_ERR_NONFINITE;
+        }
+    c->mlp_loaded = true;
+    c->ema_seeded = c->ema_plain = -1.0;
+    c->ws_on = true;
+    c->ws_steps = 0;
+    return LPS_OK;
+}
+
+int lps_mlp_forward(lps_ctx* c, const float* x, uint32_t n, float* y) {
+    if (!c || !x || !y || n == 0) return LPS_ERR_INVALID;
+    if (!c->mlp_loaded) return LPS_ERR_STATE;
+    float *dx = nullptr, *dy = nullptr;
+    int st = dalloc(&dx, (size_t)n * LPS_MLP_IN);
+    if (!st) st = dalloc(&dy, n);
+    if (!st && hipMemcpy(dx, x, sizeof(float) * LPS_MLP_IN * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
+        st = LPS_ERR_HIP;
+    if (!st) {
+        lps_dev::launch_mlp_forward(c->stream, mlp_args(c), dx, n, dy);
+        if (hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            st = LPS_ERR_HIP;
+    }
+    dfree(dx);
+    dfree(dy);
+    return st;
+}
+
+int lps_mlp_grad(lps_ctx* c, const float* x, const float* t, uint32_t n, float over_weight, float* grad,
+                 float* loss) {
+    if (!c || !x || !t || !grad || n == 0 || !(over_weight > 0.f) || !std::isfinite(over_weight))
+        return LPS_ERR_INVALID;
+    if (!c->mlp_loaded) return LPS_ERR_STATE;
+    int st = ensure_train_buffers(c);
+    if (st) return st;
+    float *dx = nullptr, *dt = nullptr;
+    if ((st = dalloc(&dx, (size_t)n * LPS_MLP_IN)) || (st = dalloc(&dt, n))) {
+        dfree(dx);
+        return st;
+    }
+    const hipStream_t s = c->stream;
+    if (hipMemcpy(dx, x, sizeof(float) * LPS_MLP_IN * (size_t)n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(dt, t, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(c->d_grad, 0, sizeof(float) * LPS_MLP_PARAMS, s) != hipSuccess ||
+        hipMemsetAsync(c->d_loss, 0, sizeof(float), s) != hipSuccess)
+        st = LPS_ERR_HIP;
+    if (!st) {
+        const lps_dev::RowMap rm{nullptr, 0u, 1u, 0u, n};
+        lps_dev::launch_mlp_grad(s, mlp_args(c), dx, dt, rm, n, over_weight, c->d_grad, c->d_loss);
+        float l = 0.f;
+        if (hipMemcpyAsync(grad, c->d_grad, sizeof(float) * LPS_MLP_PARAMS, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(&l, c->d_loss, sizeof(float), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemsetAsync(c->d_grad, 0, sizeof(float) * LPS_MLP_PARAMS, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            st = LPS_ERR_HIP;
+        if (loss) *loss = l;
+    }
+    dfree(dx);
+    dfree(dt);
+    return st;
 }
 
 int lps_profile_enable(lps_ctx* c, int32_t on) {

@@ -17,6 +17,7 @@
 #define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)
 #define LPS_MAX_ITER_SLOTS 64   // per-iteration error slots (iterations beyond reuse the last)
 #define LPS_MAX_COLLIDERS 64
+#define LPS_FEAT 16              // warm-start features per particle (LPS_MLP_IN)
 
 // Device-resident grid state of the current step.  Written by the bbox
 // reduction (integrate of the previous step, or the bbox kernel) and by the

@@ -59,10 +59,43 @@ void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint3
                        uint32_t cell_cap);
 void launch_density(hipStream_t s, const float4* pos, const DevGrid* g, const uint32_t* cell_lb, const SimConst& c,
                     const float* tables, float4* posrho, float* press, float* q);
-void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred);
+void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat);
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, const LoopRule& rule, uint32_t* host_flag);
+                        float* q, uint32_t k, const LoopRule& rule, float* feat, uint32_t* host_flag);
+
+// ---- learned warm start (lps_mlp.hip)
+// Network parameters, fp32 master copy in this order: W1 [64][16], b1 [64],
+// W2 [64][64], b2 [64], W3 [64], b3 [1]; features are normalised as
+// (f - mean) * inv_std before the first layer, the output scaled by tscale.
+struct MlpArgs {
+    const float* params;
+    const float* fscale;  // mean[16], inv_std[16]
+    float tscale;
+};
+// Row of batch sample s: idx[s] when idx is given, else the affine
+// permutation ((start + s) * mul + add) mod rows (mul coprime to rows).
+struct RowMap {
+    const uint32_t* idx;
+    uint64_t start, mul, add, rows;
+    __host__ __device__ uint32_t row(uint32_t s) const {
+        return idx ? idx[s] : (uint32_t)(((start + s) * mul + add) % rows);
+    }
+};
+// p_i = pscale * tscale * MLP(features_i) (non-negative when negative
+// pressures are disabled, 0 when not finite), q_i = p_i / rho*_i^2,
+// seed_i = p_i.  No-op once the step has stopped or failed.
+void launch_mlp_seed(hipStream_t s, const MlpArgs& m, const float* feat, uint32_t n, float* press, float* q,
+                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g);
+// batch inference (raw output, before tscale) / gradients, for training and tests
+void launch_mlp_forward(hipStream_t s, const MlpArgs& m, const float* x, uint32_t n, float* y);
+void launch_mlp_grad(hipStream_t s, const MlpArgs& m, const float* x, const float* t, const RowMap& rm, uint32_t n,
+                     float over_weight, float* grad, float* loss);
+void launch_adam(hipStream_t s, float* params, float* grad, float* m1, float* m2, uint32_t count, float lr,
+                 float b1, float b2, float eps, uint32_t step);
+void launch_feature_stats(hipStream_t s, const float* x, uint64_t n, uint32_t cols, double* acc);
+void launch_collect(hipStream_t s, const float* feat, const float* press, float pscale, uint32_t n, uint32_t stride,
+                    uint32_t offset, uint32_t count, float* X, float* T, uint64_t base);
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
                          const float* q, float4* fp, float4* xpred_next, uint32_t k);
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,

@@ -712,12 +712,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
 // accumulateNonPressureForces (SPHKernels.cl:256-349); for PCISPH also the
 // first prediction of the correction loop (pciInit + pciIntegrate +
 // pciHandleCollisions with fp = 0, SPHKernels.cl:661-735).
+template <bool FEAT>
 __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c,
     const float* __restrict__ tables, const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres,
     const lps_plane* __restrict__ planes, float4* __restrict__ force, float4* __restrict__ xpred,
-    uint32_t nlogical) {
+    const float* __restrict__ pprev, float* __restrict__ feat, uint32_t nlogical) {
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
@@ -734,6 +735,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     const float ri = xi.w;
     float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
+    float nb = 0.f, dv = 0.f;  // warm-start features: neighbour count, velocity divergence
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max,
                          [](uint32_t) { return true; }, [&](uint32_t j) {
         const float4 xj = posrho[j];
@@ -752,10 +754,28 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
             vfx += (vj.x - vi.x) * wv / rj;
             vfy += (vj.y - vi.y) * wv / rj;
             vfz += (vj.z - vi.z) * wv / rj;
+            if constexpr (FEAT) {
+                nb += 1.f;
+                dv += ((vj.x - vi.x) * (xi.x - xj.x) + (vj.y - vi.y) * (xi.y - xj.y) + (vj.z - vi.z) * (xi.z - xj.z)) *
+                      w1 / rj;
+            }
         }
     });
     snx *= c.mass; sny *= c.mass; snz *= c.mass;
     const float snl = sqrtf(snx * snx + sny * sny + snz * snz);
+    if constexpr (FEAT) {
+        float* fr = feat + (size_t)i * LPS_FEAT;
+        fr[0] = ri / c.rest_density - 1.f;
+        fr[1] = nb * (1.f / 32.f);
+        fr[2] = dv * c.mass * c.dt;
+        fr[3] = snx * c.h;
+        fr[4] = sny * c.h;
+        fr[5] = snz * c.h;
+        fr[6] = snl * c.h;
+        fr[7] = pprev[i] / (c.delta * c.rest_density);
+        fr[8] = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * c.dt / c.h;
+        fr[9] = first[i] ? 1.f : 0.f;
+    }
     if (snl > c.st_threshold) {
         const float k = -c.surface_tension * lc * c.mass;
         fx += (snx / snl) * k;
@@ -768,7 +788,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     force[i] = make_float4(fx, fy, fz, 0.f);
     if (xpred) {
         const float4 xo = make_float4(xi.x, xi.y, xi.z, 0.f);
-        xpred[i] = predict(xo, vi, vh[i], first[i] != 0, fx + 0.f, fy + 0.f, fz + 0.f, ri, c, boxes, spheres, planes);
+        const float4 xp = predict(xo, vi, vh[i], first[i] != 0, fx + 0.f, fy + 0.f, fz + 0.f, ri, c, boxes, spheres,
+                                  planes);
+        xpred[i] = xp;
+        if constexpr (FEAT) {
+            const float ex = xp.x - xo.x, ey = xp.y - xo.y, ez = xp.z - xo.z;
+            feat[(size_t)i * LPS_FEAT + 10] = sqrtf(ex * ex + ey * ey + ez * ez) / c.h;
+        }
     }
 }
 
@@ -822,10 +848,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
 // `stall_limit` corrections, or at k == max; a NaN error marks the step
 // non-finite.  It writes {state, corrections, error bits, converged} to the
 // host slot (state 0 continue, 1 stopped, 2 error).
+template <bool FEAT>
 __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
-    float* __restrict__ q, uint32_t k, LoopRule rule, uint32_t* __restrict__ host_flag, uint32_t nlogical) {
+    float* __restrict__ q, uint32_t k, LoopRule rule, float* __restrict__ feat, const float4* __restrict__ posrho,
+    uint32_t* __restrict__ host_flag, uint32_t nlogical) {
     const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
     if (step_dead(g) || g->stop_iter != 0u) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -869,6 +897,15 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         rho_pred[i] = rp;
         press[i] = p;
         q[i] = p / (rp * rp);
+        if (FEAT && k == 1u) {
+            float* fr = feat + (size_t)i * LPS_FEAT;
+            const float e1 = rp / c.rest_density - 1.f;
+            fr[11] = e1;
+            fr[12] = fmaxf(e1, 0.f);
+            fr[13] = dp / (c.delta * c.rest_density);
+            fr[14] = (rp - posrho[i].w) / c.rest_density;
+            fr[15] = e1 * e1 * 100.f;
+        }
         ebits = __float_as_uint(fabsf(e));
     }
     ebits = wave_max_u32(ebits);
@@ -890,6 +927,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     uint32_t state = 0;
     if (eb >= 0x7F800000u) {
         g->status = LPS_ERR_NONFINITE;
+        g->steps_ok -= 1u;  // counted by the grid setup; this step did not complete
         state = 2;
     } else {
         // non-negative floats order like their bit patterns
@@ -1063,10 +1101,11 @@ void launch_density(hipStream_t s, const float4* pos, const DevGrid* g, const ui
     k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, g, cell_lb, c, tables, posrho, press, q, nl);
 }
 
-void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred) {
+void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    k_forces<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables,
-                                                a.boxes, a.spheres, a.planes, a.force, xpred, nl);
+    auto kern = feat ? k_forces<true> : k_forces<false>;
+    kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables, a.boxes,
+                                            a.spheres, a.planes, a.force, xpred, pprev, feat, nl);
 }
 
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
@@ -1075,10 +1114,11 @@ void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
 }
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, const LoopRule& rule, uint32_t* host_flag) {
+                        float* q, uint32_t k, const LoopRule& rule, float* feat, uint32_t* host_flag) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    k_pci_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q,
-                                                     k, rule, host_flag, nl);
+    auto kern = (feat && k == 1u) ? k_pci_density<true> : k_pci_density<false>;
+    kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, rule,
+                                            feat, a.posrho, host_flag, nl);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,

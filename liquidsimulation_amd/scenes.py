@@ -96,6 +96,11 @@ class Scene:
     def n(self) -> int:
         return int(self.x.shape[0])
 
+    def with_params(self, **kw) -> "Scene":
+        """The same scene with some fluid/solver parameters replaced."""
+        return Scene(name=self.name, params=self.params.replace(**kw), colliders=self.colliders, x=self.x,
+                     v=self.v, dt=self.dt, steps=self.steps, meta=dict(self.meta))
+
 
 def _fluid_params(s: float, method: int, **kw) -> Params:
     # r_p = s / 1.6 so the reference's mass lattice step 1.6 r_p equals s;
@@ -143,6 +148,13 @@ def sph_dam_break_16k(seed: int = 1) -> Scene:
     dt 0.002 s, 500 steps, +-1e-4 m jitter."""
     return dam_break(20, 40, 20, 0.02, jitter_amp=1e-4, seed=seed, method=METHOD_SPH, dt=0.002,
                      steps=500, name="sph_dam_break_16k")
+
+
+def pcisph_dam_break_16k(seed: int = 1) -> Scene:
+    """The 16,000-particle dam break (20x40x20 at 0.02 m, dt 0.002 s) run with
+    PCISPH at the 524k scene's tolerances (1 %, 3..50 corrections)."""
+    return dam_break(20, 40, 20, 0.02, jitter_amp=1e-4, seed=seed, method=METHOD_PCISPH, dt=0.002,
+                     steps=500, name="pcisph_dam_break_16k")
 
 
 def pcisph_dam_break_524k(seed: int = 2) -> Scene:
@@ -202,6 +214,7 @@ def small_block(nx=8, ny=8, nz=8, s=0.02, jitter_amp=1e-4, seed=5, method=METHOD
 
 WORKLOADS = {
     "sph_dam_break_16k": sph_dam_break_16k,
+    "pcisph_dam_break_16k": pcisph_dam_break_16k,
     "pcisph_dam_break_524k": pcisph_dam_break_524k,
     "pcisph_obstacles_2m": pcisph_obstacles_2m,
 }

@@ -17,7 +17,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib
-from .abi import (CBox, CGridInfo, CParams, CPlane, CSphere, CStepStats, Box, Colliders, Params, Plane,
+from .abi import (CBox, CGridInfo, CParams, CPlane, CSphere, CStepStats, CTrainCfg, Box, Colliders, Params, Plane,
                   Sphere, LPS_TABLE_SIZE, LPS_MLP_IN, LPS_MLP_PARAMS, METHOD_PCISPH, METHOD_SPH)
 from ._lib import check, lib, ptr
 
@@ -280,9 +280,69 @@ class SPHSolver:
 
 
 class PCISPHSolver(SPHSolver):
-    """PCISPH solver (reference: PCISPHSolver, PCISPHSolver.cpp:50-240)."""
+    """PCISPH solver (reference: PCISPHSolver, PCISPHSolver.cpp:50-240), plus
+    the learned pressure warm start (no counterpart in the reference)."""
 
     _default_method = METHOD_PCISPH
+
+    # ---- learned warm start
+    def set_warm_start_weights(self, params, feature_scale, target_scale: float):
+        params = np.ascontiguousarray(params, np.float32).reshape(LPS_MLP_PARAMS)
+        feature_scale = np.ascontiguousarray(feature_scale, np.float32).reshape(2 * LPS_MLP_IN)
+        check(lib().lps_mlp_set_weights(self._ctx, params, feature_scale, float(target_scale)),
+              "lps_mlp_set_weights")
+
+    def get_warm_start_weights(self):
+        w = np.zeros(LPS_MLP_PARAMS, np.float32)
+        fs = np.zeros(2 * LPS_MLP_IN, np.float32)
+        ts = C.c_float()
+        check(lib().lps_mlp_get_weights(self._ctx, w, fs, C.byref(ts)), "lps_mlp_get_weights")
+        return {"params": w, "feature_scale": fs, "target_scale": ts.value}
+
+    def capture_features(self, on: bool = True):
+        check(lib().lps_mlp_capture(self._ctx, 1 if on else 0), "lps_mlp_capture")
+
+    def dump_pairs(self, seed: bool = False):
+        """Features [n, 16], final pressure [n] (and the seed [n]) of the last step."""
+        n = self._n
+        x = np.zeros((n, LPS_MLP_IN), np.float32)
+        t = np.zeros(n, np.float32)
+        s = np.zeros(n, np.float32) if seed else None
+        check(lib().lps_dump_pairs(self._ctx, ptr(x), ptr(t), ptr(s), n), "lps_dump_pairs")
+        return (x, t, s) if seed else (x, t)
+
+    def collect(self, capacity: int, stride: int = 1):
+        check(lib().lps_mlp_collect(self._ctx, int(capacity), int(stride)), "lps_mlp_collect")
+
+    def collected(self) -> int:
+        r = C.c_uint64()
+        check(lib().lps_mlp_collected(self._ctx, C.byref(r)), "lps_mlp_collected")
+        return r.value
+
+    def train_warm_start(self, epochs: int, batch: int = 65536, lr: float = 2e-3, lr_final: float = 1e-5,
+                         beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, reinit: bool = True,
+                         refit_scale: bool = True, over_weight: float = 1.0, seed: int = 0):
+        cfg = CTrainCfg(epochs=int(epochs), batch=int(batch), lr=lr, lr_final=lr_final, beta1=beta1, beta2=beta2,
+                        eps=eps, reinit=int(reinit), refit_scale=int(refit_scale), over_weight=float(over_weight),
+                        seed=int(seed))
+        loss = np.zeros(max(int(epochs), 1), np.float32)
+        check(lib().lps_mlp_train(self._ctx, C.byref(cfg), loss.ctypes.data), "lps_mlp_train")
+        return loss[: int(epochs)]
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, LPS_MLP_IN)
+        y = np.zeros(x.shape[0], np.float32)
+        check(lib().lps_mlp_forward(self._ctx, x.reshape(-1), x.shape[0], y), "lps_mlp_forward")
+        return y
+
+    def mlp_grad(self, x, t, over_weight: float = 1.0):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, LPS_MLP_IN)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        g = np.zeros(LPS_MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().lps_mlp_grad(self._ctx, x.reshape(-1), t, x.shape[0], float(over_weight), g, C.byref(loss)),
+              "lps_mlp_grad")
+        return g, loss.value
 
     def setMinIterations(self, k: int):
         self._apply(min_iterations=int(k))

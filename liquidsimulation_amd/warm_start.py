@@ -1,0 +1,143 @@
+"""Learned PCISPH pressure warm start: weights file and training driver.
+
+The reference starts every PCISPH step from zero pressure (pciInit,
+LiquidPhysics/cl_kernels/SPHKernels.cl:661-672; PCISPHSolver.cpp:56-64).  The
+library can instead seed the first correction's pressure with a per-particle
+MLP (16 -> 64 -> 64 -> 1, BF16 on the matrix cores); the correction loop and
+its tolerance are unchanged, so the seed only changes how many corrections a
+step needs, and the library falls back to the zero start when the seed does
+not pay (include/lps.h documents features and rule).
+
+Training runs on the device with the library's own kernels: the solver runs
+training scenes with collection on (lps_mlp_collect appends rows of
+(features, final pressure) in HBM), then lps_mlp_train runs Adam over them.
+A second round collects with the trained seed switched on, so the network also
+sees the states its own seeding produces, and continues training on both.
+
+    python -m liquidsimulation_amd.warm_start --out liquidsimulation_amd/weights/warm_start_dam_break.npz
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from typing import Callable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from .abi import LPS_MLP_IN, LPS_MLP_PARAMS, METHOD_PCISPH, WARM_LEARNED, WARM_NONE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "weights", "warm_start_dam_break.npz")
+
+
+def load(path: str = WEIGHTS) -> dict:
+    with np.load(path, allow_pickle=False) as z:
+        w = {"params": z["params"].astype(np.float32), "feature_scale": z["feature_scale"].astype(np.float32),
+             "target_scale": float(z["target_scale"])}
+        w["meta"] = json.loads(str(z["meta"])) if "meta" in z.files else {}
+    if w["params"].shape != (LPS_MLP_PARAMS,) or w["feature_scale"].shape != (2 * LPS_MLP_IN,):
+        raise ValueError(f"{path}: unexpected shapes")
+    return w
+
+
+def save(path: str, w: dict, meta: Optional[dict] = None) -> None:
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=np.asarray(w["params"], np.float32),
+             feature_scale=np.asarray(w["feature_scale"], np.float32),
+             target_scale=np.float32(w["target_scale"]), meta=json.dumps(meta or {}))
+
+
+def apply(solver, w: Optional[dict] = None) -> None:
+    """Load weights (default: the committed file) into a PCISPH solver."""
+    w = load() if w is None else w
+    solver.set_warm_start_weights(w["params"], w["feature_scale"], w["target_scale"])
+
+
+def _run(solver, scene, steps: int, collect_from: int, stride: int, capacity: int, log) -> dict:
+    solver.set_params(scene.params)
+    solver.set_colliders(scene.colliders)
+    solver.set_particles(scene.x, scene.v)
+    its: List[int] = []
+    used = 0
+    chunk = 50
+    done = 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        on = done + k > collect_from
+        solver.collect(capacity, stride if on else 0)
+        st = solver.update(scene.dt, k)
+        its += [x.iterations for x in st]
+        used += sum(x.warm_start_used for x in st)
+        done += k
+    solver.collect(capacity, 0)
+    r = {"scene": scene.name, "steps": steps, "mean_iterations": float(np.mean(its)), "seeded_steps": used,
+         "rows": solver.collected()}
+    log(f"  {scene.name}: {steps} steps, mean corrections {r['mean_iterations']:.2f}, "
+        f"seeded {used}, rows {r['rows']}")
+    return r
+
+
+def train(scenes: Sequence, steps: int = 2000, stride: int = 40, capacity: int = 60_000_000,
+          epochs: Tuple[int, int] = (6, 4), batch: int = 65536, lr: float = 2e-3, seed: int = 0,
+          over_weight: float = 3.0, dagger: bool = True, log: Callable = print) -> Tuple[dict, dict]:
+    """Collect on `scenes` (each run `steps` steps, every `stride`-th particle
+    per step), train, then (dagger) collect again with the seed on and keep
+    training.  Returns (weights, metadata)."""
+    from .solver import PCISPHSolver
+
+    t0 = time.perf_counter()
+    base = scenes[0].params.replace(warm_start=WARM_NONE)
+    s = PCISPHSolver(base)
+    s.collect(capacity, stride)
+    meta = {"scenes": [sc.name for sc in scenes], "steps": steps, "stride": stride, "batch": batch, "lr": lr,
+            "seed": seed, "over_weight": over_weight, "runs": []}
+    log("collect (zero start):")
+    for sc in scenes:
+        meta["runs"].append(_run(s, sc.with_params(warm_start=WARM_NONE), steps, 0, stride, capacity, log))
+    loss1 = s.train_warm_start(epochs[0], batch=batch, lr=lr, lr_final=lr * 0.01, reinit=True, refit_scale=True,
+                               over_weight=over_weight, seed=seed)
+    log(f"train: {s.collected()} rows, {epochs[0]} epochs, loss {loss1[0]:.4g} -> {loss1[-1]:.4g}")
+    meta["loss_round1"] = [float(v) for v in loss1]
+    if dagger:
+        log("collect (learned start):")
+        for sc in scenes:
+            meta["runs"].append(_run(s, sc.with_params(warm_start=WARM_LEARNED), steps, 0, stride, capacity, log))
+        loss2 = s.train_warm_start(epochs[1], batch=batch, lr=lr * 0.5, lr_final=lr * 0.005, reinit=False,
+                                   refit_scale=False, over_weight=over_weight, seed=seed + 1)
+        log(f"train: {s.collected()} rows, {epochs[1]} epochs, loss {loss2[0]:.4g} -> {loss2[-1]:.4g}")
+        meta["loss_round2"] = [float(v) for v in loss2]
+    w = s.get_warm_start_weights()
+    meta["rows"] = int(s.collected())
+    meta["seconds"] = time.perf_counter() - t0
+    s.close()
+    return w, meta
+
+
+def main(argv=None):
+    from . import scenes as S
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=WEIGHTS)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--stride", type=int, default=40)
+    ap.add_argument("--epochs", type=int, nargs=2, default=(6, 4))
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--over-weight", type=float, default=3.0)
+    ap.add_argument("--no-dagger", action="store_true")
+    args = ap.parse_args(argv)
+    # training scenes: the 524k dam break and the 16k dam break run as PCISPH,
+    # with jitter seeds other than the benchmark's
+    scenes = [S.pcisph_dam_break_524k(seed=11), S.pcisph_dam_break_16k(seed=12)]
+    w, meta = train(scenes, steps=args.steps, stride=args.stride, epochs=tuple(args.epochs), batch=args.batch,
+                    lr=args.lr, seed=args.seed, over_weight=args.over_weight, dagger=not args.no_dagger)
+    save(args.out, w, meta)
+    print(json.dumps({k: v for k, v in meta.items() if k != "runs"}), flush=True)
+    print(f"wrote {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

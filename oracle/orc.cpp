@@ -530,7 +530,7 @@ inline V3<T> cell_anchor(const std::vector<V3<float>>& xf, int32_t i) {
 
 template <class T>
 int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* x, T* v, T* vh,
-             uint8_t* first, orc_trace* tr, int nthreads) {
+             uint8_t* first, orc_trace* tr, int nthreads, const float* seed = nullptr) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -731,6 +731,22 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
                 s.rp[i] = rp;
                 pp[i] += dp;
                 err[i] = std::fabs(e);
+            }
+            if (seed && k == 1) {
+                // learned warm start (no reference counterpart): unless the
+                // loop stops at the first correction, its pressure is replaced
+                // by the given seed (sorted order) before the pressure force
+                T mx1 = 0;
+                bool nf1 = false;
+                for (int32_t i = 0; i < n; i++) {
+                    if (std::isnan(err[i])) nf1 = true;
+                    else mx1 = std::max(mx1, err[i]);
+                }
+                float r1 = (float)mx1 / p->rest_density;
+                if (sizeof(T) == sizeof(double)) r1 = (float)(mx1 / rho0);
+                const bool stop1 = nf1 || (k >= minit && r1 < p->max_density_error_ratio) || k >= maxit;
+                if (!stop1)
+                    for (int32_t i = 0; i < n; i++) pp[i] = (T)seed[i];
             }
             // pciCalcPressureForce (SPHKernels.cl:806-878)
 #pragma omp parallel for schedule(dynamic, 256)
@@ -1168,6 +1184,15 @@ int orc_step_ocl_f32(const orc_params* p, const orc_box* boxes, int32_t nbox, co
                      float* v, float* vh, uint8_t* first, orc_trace* tr, int32_t nthreads) {
     Colliders cl{boxes, nbox, spheres, nsphere, planes, nplane};
     return step_ocl<float>(p, cl, dt, n, x, v, vh, first, tr, nthreads);
+}
+
+int orc_step_ocl_seeded_f32(const orc_params* p, const orc_box* boxes, int32_t nbox, const orc_sphere* spheres,
+                            int32_t nsphere, const orc_plane* planes, int32_t nplane, float dt, int32_t n, float* x,
+                            float* v, float* vh, uint8_t* first, const float* seed, orc_trace* tr,
+                            int32_t nthreads) {
+    if (!seed) return ST_INVALID;
+    Colliders cl{boxes, nbox, spheres, nsphere, planes, nplane};
+    return step_ocl<float>(p, cl, dt, n, x, v, vh, first, tr, nthreads, seed);
 }
 
 int orc_step_ocl_f64(const orc_params* p, const orc_box* boxes, int32_t nbox, const orc_sphere* spheres,

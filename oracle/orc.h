@@ -110,6 +110,14 @@ int orc_step_ocl_f32(const orc_params* p, const orc_box* boxes, int32_t nbox,
                      const orc_plane* planes, int32_t nplane,
                      float dt, int32_t n, float* x, float* v, float* vh, uint8_t* first,
                      orc_trace* tr, int32_t nthreads);
+/* the same step with the first correction's pressure replaced by seed[n]
+ * (sorted order of this step) unless the loop stops at the first correction:
+ * the learned warm start's loop, with the network's output as given input */
+int orc_step_ocl_seeded_f32(const orc_params* p, const orc_box* boxes, int32_t nbox,
+                            const orc_sphere* spheres, int32_t nsphere,
+                            const orc_plane* planes, int32_t nplane,
+                            float dt, int32_t n, float* x, float* v, float* vh, uint8_t* first,
+                            const float* seed, orc_trace* tr, int32_t nthreads);
 int orc_step_ocl_f64(const orc_params* p, const orc_box* boxes, int32_t nbox,
                      const orc_sphere* spheres, int32_t nsphere,
                      const orc_plane* planes, int32_t nplane,

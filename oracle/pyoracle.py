@@ -80,6 +80,8 @@ def lib():
                      C.c_float, C.c_int32]
         L.orc_step_ocl_f32.argtypes = step_args + [_f32p, _f32p, _f32p, _u8p, C.POINTER(COrcTrace), C.c_int32]
         L.orc_step_ocl_f64.argtypes = step_args + [_f64p, _f64p, _f64p, _u8p, C.POINTER(COrcTrace), C.c_int32]
+        L.orc_step_ocl_seeded_f32.argtypes = step_args + [_f32p, _f32p, _f32p, _u8p, C.c_void_p,
+                                                          C.POINTER(COrcTrace), C.c_int32]
         L.orc_step_seq_f32.argtypes = [C.POINTER(CParams), C.c_void_p, C.c_void_p, C.c_void_p, _i32p, _i32p,
                                        C.c_int32, C.c_float, C.c_int32, _f32p, _f32p, _f32p, _u8p,
                                        C.POINTER(COrcTrace)]
@@ -244,6 +246,22 @@ def step_ocl(params: Params, col: Colliders, dt: float, st: State, nthreads: int
     fn = lib().orc_step_ocl_f64 if f64 else lib().orc_step_ocl_f32
     x, v, vh = (a.reshape(-1) for a in (st.x, st.v, st.vh))
     fn(C.byref(cp), *cargs, dt, n, x, v, vh, st.first, C.byref(tr), nthreads)
+    del keep
+    return _read_trace(tr, bufs, n)
+
+
+def step_ocl_seeded(params: Params, col: Colliders, dt: float, st: State, seed: np.ndarray, nthreads: int = 0,
+                    cell_cap: int = 1 << 22) -> Trace:
+    """step_ocl with the first correction's pressure replaced by `seed` (sorted
+    order of this step) unless the loop stops at the first correction."""
+    n = st.n
+    seed = np.ascontiguousarray(seed, np.float32).reshape(n)
+    tr, bufs = _mk_trace(n, cell_cap, np.float32)
+    cp = params.to_c()
+    keep, cargs = _col_ptrs(col)
+    x, v, vh = (a.reshape(-1) for a in (st.x, st.v, st.vh))
+    lib().orc_step_ocl_seeded_f32(C.byref(cp), *cargs, dt, n, x, v, vh, st.first, seed.ctypes.data,
+                                  C.byref(tr), nthreads)
     del keep
     return _read_trace(tr, bufs, n)
 

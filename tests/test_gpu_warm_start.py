@@ -1,0 +1,257 @@
+"""GPU tests of the learned PCISPH warm start.
+
+* the BF16 MLP kernels (inference, gradient) against the CPU restatement in
+  oracle/mlp.py, which emulates their roundings; tolerances are stated per test;
+* device training lowers the loss;
+* a seeded step equals the oracle's step given the same seed, bit for bit
+  (the unchanged correction loop judges the seed);
+* the fallback: a harmful seed is switched off;
+* the committed weights reduce corrections per step on the dam break.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from liquidsimulation_amd import scenes
+from liquidsimulation_amd.abi import METHOD_PCISPH, WARM_LEARNED, WARM_NONE, Params
+from liquidsimulation_amd.solver import PCISPHSolver
+from oracle import mlp as M
+from tests.helpers import ulp_diff
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_net(seed):
+    rng = np.random.default_rng(seed)
+    p = M.he_init(seed)
+    p[M.IN * M.HID:M.IN * M.HID + M.HID] = rng.normal(0, 0.1, M.HID)
+    o2 = M.IN * M.HID + M.HID + M.HID * M.HID
+    p[o2:o2 + M.HID] = rng.normal(0, 0.1, M.HID)
+    p[-1] = 0.25
+    fs = np.concatenate([rng.normal(0, 1, M.IN), rng.uniform(0.5, 2.0, M.IN)]).astype(np.float32)
+    return p.astype(np.float32), fs
+
+
+def _ctx():
+    return PCISPHSolver(Params(method=METHOD_PCISPH))
+
+
+@pytest.mark.parametrize("n", [1, 13, 1000, 70001])
+def test_mlp_forward_matches_oracle(gpu, n):
+    """GPU output vs the BF16-emulating oracle: the only difference is the FP32
+    summation order inside the matrix cores (and the rare BF16 rounding of an
+    activation it flips), so rms error <= 1e-4 and max <= 4e-3 of rms(y).  The
+    exact (unrounded) network is ~1e-3 away: the kernels do round to BF16."""
+    p, fs = _random_net(7)
+    s = _ctx()
+    s.set_warm_start_weights(p, fs, 1.0)
+    x = np.random.default_rng(n).normal(0, 1.5, (n, M.IN)).astype(np.float32) * fs[M.IN:] ** -1 + fs[:M.IN]
+    y = s.mlp_forward(x).astype(np.float64)
+    ref = M.forward(p, fs, x)["y"]
+    exact = M.forward(p, fs, x, exact=True)["y"]
+    scale = np.sqrt(np.mean(ref ** 2))
+    err = np.abs(y - ref)
+    print(f"n={n} rms(y)={scale:.4g} rms err={np.sqrt(np.mean(err**2))/scale:.3g} max={err.max()/scale:.3g} "
+          f"vs exact {np.sqrt(np.mean((y-exact)**2))/scale:.3g}", flush=True)
+    assert np.sqrt(np.mean(err ** 2)) <= 1e-4 * scale
+    assert err.max() <= 4e-3 * scale
+    if n >= 1000:
+        assert np.sqrt(np.mean((y - exact) ** 2)) > 10 * np.sqrt(np.mean(err ** 2))
+    s.close()
+
+
+@pytest.mark.parametrize("n,ow", [(17, 1.0), (3001, 1.0), (65536, 1.0), (3001, 3.0)])
+def test_mlp_gradient_matches_oracle(gpu, n, ow):
+    """Gradient of mean((y - t/tscale)^2): per parameter block, the L2 error of
+    the GPU gradient relative to the oracle's is <= 2e-3 (FP32 atomics in any
+    order, BF16 roundings of the back-propagated terms flipped by them), and
+    the loss agrees to 1e-4."""
+    p, fs = _random_net(11)
+    ts = 2.5
+    s = _ctx()
+    s.set_warm_start_weights(p, fs, ts)
+    rng = np.random.default_rng(n)
+    x = (rng.normal(0, 1.5, (n, M.IN)) / fs[M.IN:] + fs[:M.IN]).astype(np.float32)
+    t = rng.normal(0, 3.0, n).astype(np.float32)
+    g, loss = s.mlp_grad(x, t, ow)
+    gref, lref = M.grad(p, fs, ts, x, t, over_weight=ow)
+    assert abs(loss - lref) <= 1e-4 * lref
+    o = 0
+    for name, size in (("W1", M.HID * M.IN), ("b1", M.HID), ("W2", M.HID * M.HID), ("b2", M.HID),
+                       ("W3", M.HID), ("b3", 1)):
+        a, b = g[o:o + size].astype(np.float64), gref[o:o + size]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        print(f"n={n} {name}: rel L2 err {rel:.3g}", flush=True)
+        assert rel <= 2e-3, name
+        o += size
+    s.close()
+
+
+def test_device_training_lowers_loss(gpu):
+    """Collect (features -> final pressure) rows from a small PCISPH dam break
+    and train on the device: the loss falls well below its starting value."""
+    sc = scenes.small_block(16, 24, 16, dt=0.002)
+    s = PCISPHSolver(sc.params)
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x, sc.v)
+    s.collect(2_000_000, 1)
+    s.update(sc.dt, 60)
+    rows = s.collected()
+    assert rows == 60 * sc.n
+    loss = s.train_warm_start(8, batch=8192, lr=2e-3, seed=1)
+    print("loss per epoch:", loss, flush=True)
+    assert np.all(np.isfinite(loss))
+    assert loss[-1] < 0.35 * loss[0]
+    w = s.get_warm_start_weights()
+    assert np.all(np.isfinite(w["params"])) and w["target_scale"] > 0
+    s.close()
+
+
+def _features_check(x, t, sc, state, s):
+    """Features that follow from the state alone, checked on the host."""
+    h = s.getKernelRadius()
+    n = state["x"].shape[0]
+    # 8: |v| dt / h ; 9: first-step flag
+    vn = np.sqrt(np.sum(state["v"].astype(np.float64) ** 2, 1)) * sc.dt / h
+    np.testing.assert_allclose(x[:, 8], vn, rtol=1e-5, atol=1e-9)
+    np.testing.assert_array_equal(x[:, 9], state["first"].astype(np.float32))
+    # 1: neighbours within h (self excluded) / 32, brute force over the state
+    xs = state["x"].astype(np.float64)
+    d2 = np.sum((xs[:, None, :] - xs[None, :, :]) ** 2, -1)
+    nb = (np.sqrt(d2) < h).sum(1) - 1
+    assert np.mean(np.abs(x[:, 1] * 32 - nb) <= 1) > 0.99
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(t))
+
+
+def test_seeded_step_matches_oracle(gpu, oracle):
+    """The seeded step is the oracle's step with the same seed: state, pressure
+    and corrections bit-identical; the seed equals the network on the dumped
+    features (to the forward test's tolerance)."""
+    sc = scenes.small_block(12, 16, 10, dt=0.002)
+    p, fs = _random_net(3)
+    s = PCISPHSolver(sc.params.replace(warm_start=WARM_LEARNED))
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x, sc.v)
+    # a few zero-start steps first (no weights loaded: the seed is not used)
+    st = s.update(sc.dt, 4)
+    assert all(x.warm_start_used == 0 for x in st)
+    s.set_warm_start_weights(p, fs, 0.05)
+    for step in range(3):
+        before = s.get_particles()
+        a = oracle.State.from_scene(before["x"], before["v"])
+        a.vh[:] = before["vhalf"]
+        a.first[:] = before["first"]
+        stats = s.update(sc.dt, 1)
+        if not stats[0].warm_start_used:
+            continue
+        feat, target, seed = s.dump_pairs(seed=True)
+        tr = oracle.step_ocl_seeded(sc.params, sc.colliders, sc.dt, a, seed)
+        after = s.get_particles()
+        f = s.get_fields()
+        print(f"step {step}: its gpu/orc {stats[0].iterations}/{tr.iterations} "
+              f"seed range [{seed.min():.4g}, {seed.max():.4g}]", flush=True)
+        assert stats[0].iterations == tr.iterations
+        assert stats[0].converged == tr.converged
+        for k, u, v in (("x", after["x"], a.x), ("v", after["v"], a.v), ("vh", after["vhalf"], a.vh),
+                        ("pressure", f["pressure"], tr.pressure), ("target", target, tr.pressure)):
+            assert ulp_diff(u, v) == 0, k
+        y = M.forward(p, fs, feat)["y"] * 0.05 * (s.delta(sc.dt) * sc.params.rest_density)
+        ref = np.maximum(y, 0.0)
+        scale = np.sqrt(np.mean(y ** 2))
+        assert np.sqrt(np.mean((seed - ref) ** 2)) <= 1e-4 * scale
+        _features_check(feat, target, sc, before_sorted(before, tr), s)
+    s.close()
+
+
+def before_sorted(before, tr):
+    """The pre-step state in this step's sorted order (tr.perm maps sorted
+    slot -> incoming index)."""
+    perm = tr.perm.astype(np.int64)
+    return {k: before[k][perm] for k in ("x", "v", "first")}
+
+
+def _expected_schedule(its, period=32, alpha=0.3):
+    """The library's seeding rule restated: the first step after loading
+    weights is seeded, the next is not; from then on the seed is used while
+    its running mean of corrections does not exceed the zero start's, and every
+    period-th step runs the other mode."""
+    ema_s = ema_p = -1.0
+    on, out = True, []
+    for k, it in enumerate(its):
+        if ema_s < 0:
+            seeded = True
+        elif ema_p < 0:
+            seeded = False
+        else:
+            seeded = (not on) if k % period == period - 1 else on
+        out.append(seeded)
+        if seeded:
+            ema_s = it if ema_s < 0 else (1 - alpha) * ema_s + alpha * it
+        else:
+            ema_p = it if ema_p < 0 else (1 - alpha) * ema_p + alpha * it
+        on = ema_p < 0 or ema_s < 0 or ema_s <= ema_p
+    return np.array(out, np.int32)
+
+
+def test_harmful_seed_is_switched_off(gpu):
+    """A seed far above the converged pressure, with negative pressures enabled
+    (so the loop must pull it back down), costs corrections; the library
+    measures that and returns to the zero start, probing the seed only every
+    32nd step.  The schedule equals the restated rule step for step.
+    Non-finite weights are refused."""
+    sc = scenes.small_block(12, 16, 10, dt=0.002, negative_pressure_factor=1.0)
+    p, fs = _random_net(5)
+    p[:] = 0.0
+    p[-1] = 1.0  # constant output: seed = 400 delta rho0 everywhere
+    s = PCISPHSolver(sc.params.replace(warm_start=WARM_LEARNED))
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x, sc.v)
+    s.update(sc.dt, 5)
+    s.set_warm_start_weights(p, fs, 400.0)
+    st = s.update(sc.dt, 70)
+    used = np.array([x.warm_start_used for x in st], np.int32)
+    its = np.array([x.iterations for x in st])
+    print("seeded steps:", np.nonzero(used)[0], "its seeded/plain:", its[used == 1].mean(),
+          its[used == 0].mean(), flush=True)
+    np.testing.assert_array_equal(used, _expected_schedule(its))
+    assert its[used == 1].mean() > its[used == 0].mean()
+    assert used.sum() <= 4
+    bad = p.copy()
+    bad[3] = np.nan
+    with pytest.raises(Exception):
+        s.set_warm_start_weights(bad, fs, 1.0)
+    s.close()
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "liquidsimulation_amd",
+                       "weights", "warm_start_dam_break.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained weights committed")
+def test_trained_warm_start_gains_on_dam_break(gpu):
+    """The committed network on the 524k dam break (benchmark jitter seed, not
+    a training seed), 300 steps from the start, against the zero start:
+    fewer corrections per step and more particle-steps/s, with no step worse
+    converged than the zero start allows (same tolerance, same loop)."""
+    from liquidsimulation_amd import warm_start
+    import time
+    sc = scenes.pcisph_dam_break_524k()
+    res = {}
+    for mode in (WARM_NONE, WARM_LEARNED):
+        s = PCISPHSolver(sc.params.replace(warm_start=mode))
+        s.set_colliders(sc.colliders)
+        s.set_particles(sc.x, sc.v)
+        if mode == WARM_LEARNED:
+            warm_start.apply(s)
+        s.update(sc.dt, 2)
+        t0 = time.perf_counter()
+        st = s.update(sc.dt, 300)
+        dt = time.perf_counter() - t0
+        res[mode] = {"its": float(np.mean([x.iterations for x in st])), "rate": sc.n * 300 / dt,
+                     "used": int(sum(x.warm_start_used for x in st)),
+                     "nonconv": int(sum(1 - x.converged for x in st))}
+        s.close()
+    print(res, flush=True)
+    assert res[WARM_LEARNED]["its"] < 0.85 * res[WARM_NONE]["its"]
+    assert res[WARM_LEARNED]["rate"] > 1.1 * res[WARM_NONE]["rate"]
