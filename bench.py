@@ -1,13 +1,21 @@
-"""Benchmark: particle-steps/s of the PCISPH dam break on one MI355X.
+"""Benchmark: particle-steps/s of the PCISPH dam break on one MI355X, plain and
+with the learned pressure warm start.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--mode M]
 
 Workload (default): the 524,288-particle PCISPH dam break (64x128x64 block at
 0.00625 m in a 1.6x1.0x0.8 m box, dt 0.001 s, 1 % density-error tolerance,
 3..50 corrections, +-1e-5 m seeded jitter).  A step is one PhysicSolver::update
 of the whole scene; the state stays resident in HBM during the timed region.
-value = particles * K / wall seconds of the K timed steps (HIP events on the
+Rate = particles * K / wall seconds of the K timed steps (HIP events on the
 library's stream, bracketed by device synchronisation).
+
+Both modes run the same K steps from the same initial state: plain (zero
+pressure start, as the reference) and learned (the committed network seeds
+the first correction; same loop, same tolerance).  `value` is the learned
+mode's rate when the weights file is present (--mode plain forces plain); the
+plain run, the speedup and both modes' iteration and convergence statistics
+and end-state energies are reported beside it.
 
 Prints ONE JSON line.  Extra keys: per-step PCISPH iteration statistics,
 convergence qualification, the dominant kernel's roofline (algorithmic bytes
@@ -50,11 +58,14 @@ def step_bytes(k: float, passes: int = 3) -> float:
     return 300.0 + 16.0 * passes + 108.0 * k
 
 
-def run_gpu(scene, steps: int, warmup: int, profile_window: int):
+def run_gpu(scene, steps: int, warmup: int, profile_window: int, weights=None):
     import torch
     from liquidsimulation_amd.solver import solver_for
+    from liquidsimulation_amd import warm_start
 
     s = solver_for(scene.params)
+    if weights is not None:
+        warm_start.apply(s, weights)
     s.set_colliders(scene.colliders)
     s.set_particles(scene.x, scene.v)
     if warmup:
@@ -73,22 +84,50 @@ def run_gpu(scene, steps: int, warmup: int, profile_window: int):
     its = np.array([x.iterations for x in stats], np.int64)
     errs = np.array([x.max_density_error_ratio for x in stats], np.float64)
     conv = np.array([x.converged for x in stats], np.int64)
+    used = np.array([x.warm_start_used for x in stats], np.int64)
+    st = s.get_particles()
+    m = s.getParticleMass()
+    g = np.asarray(scene.params.gravity, np.float64)
+    energy = {"kinetic_J": float(0.5 * m * np.sum(st["v"].astype(np.float64) ** 2)),
+              "potential_J": float(-m * np.sum(st["x"].astype(np.float64) @ g))}
     # dominant kernel: per-launch HIP events on the library stream over a window
     # of steps continuing from the timed state
-    s.profile(True)
-    s.profile_reset()
-    win = s.update(scene.dt, profile_window)
     kernels = {}
-    for name in ("grid_setup", "keys", "radix_pass", "reorder", "cell_table", "density", "forces",
-                 "pressure_sph", "pci_density", "pci_pressure", "integrate"):
-        n, tot = s.profile_query(name)
-        if n:
-            kernels[name] = {"launches": n, "total_ms": tot, "avg_us": 1000.0 * tot / n}
-    s.profile(False)
+    win = []
+    if profile_window > 0:
+        s.profile(True)
+        s.profile_reset()
+        win = s.update(scene.dt, profile_window)
+        for name in ("grid_setup", "keys", "radix_pass", "reorder", "cell_table", "density", "forces",
+                     "pressure_sph", "pci_density", "mlp_seed", "pci_pressure", "integrate"):
+            n, tot = s.profile_query(name)
+            if n:
+                kernels[name] = {"launches": n, "total_ms": tot, "avg_us": 1000.0 * tot / n}
+        s.profile(False)
     win_its = np.array([x.iterations for x in win], np.int64)
     s.close()
-    return {"ms": ms, "wall": wall, "its": its, "errs": errs, "conv": conv, "kernels": kernels,
-            "win_its": win_its}
+    return {"ms": ms, "wall": wall, "its": its, "errs": errs, "conv": conv, "used": used, "kernels": kernels,
+            "win_its": win_its, "energy": energy}
+
+
+def mode_summary(res, scene, steps):
+    its, errs = res["its"], res["errs"]
+    nonconv = int(np.sum(res["conv"] == 0)) if scene.params.method == 1 else 0
+    return {
+        "value": scene.n * steps / (res["ms"] / 1000.0),
+        "ms_per_step": res["ms"] / steps,
+        "iterations": {"mean": float(np.mean(its)) if its.size else 0.0,
+                       "p50": float(np.percentile(its, 50)) if its.size else 0,
+                       "p99": float(np.percentile(its, 99)) if its.size else 0,
+                       "max": int(its.max()) if its.size else 0},
+        "qualified": nonconv == 0,
+        "nonconverged_steps": nonconv,
+        "worst_density_error": float(errs.max()) if errs.size else 0.0,
+        "mean_final_density_error": float(errs.mean()) if errs.size else 0.0,
+        "seeded_steps": int(res["used"].sum()),
+        "end_energy": res["energy"],
+        "kernels_avg_us": {k: v["avg_us"] for k, v in res["kernels"].items()},
+    }
 
 
 def cpu_baseline(scene, warm: int, timed: int):
@@ -124,20 +163,30 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-warm", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--mode", choices=("both", "plain"), default="both")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the SPH step has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from liquidsimulation_amd import scenes
+    from liquidsimulation_amd import scenes, warm_start
+    from liquidsimulation_amd.abi import WARM_LEARNED, WARM_NONE
     scene = scenes.WORKLOADS[args.workload]()
+    learned = None
+    if args.mode == "both" and scene.params.method == 1 and os.path.exists(warm_start.WEIGHTS):
+        w = warm_start.load()
+        sl = scene.with_params(warm_start=WARM_LEARNED)
+        learned = mode_summary(run_gpu(sl, args.steps, args.warmup, args.profile_window, weights=w), sl,
+                               args.steps)
+    scene = scene.with_params(warm_start=WARM_NONE)
     res = run_gpu(scene, args.steps, args.warmup, args.profile_window)
+    plain = mode_summary(res, scene, args.steps)
     n = scene.n
-    value = n * args.steps / (res["ms"] / 1000.0)
-    its, errs = res["its"], res["errs"]
+    head = learned if learned is not None else plain
+    value = head["value"]
+    its = res["its"]
     tol = scene.params.max_density_error_ratio
-    nonconv = int(np.sum(res["conv"] == 0)) if scene.params.method == 1 else 0
     kernels = res["kernels"]
     dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])[0] if kernels else None
     roof = None
@@ -149,8 +198,8 @@ def main():
             roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
                     "algo_bytes_per_launch": per * n, "avg_launch_us": kernels[dom]["avg_us"]}
-    mean_k = float(np.mean(its)) if its.size else 0.0
-    step_roof = n * step_bytes(mean_k) * args.steps / (res["ms"] / 1000.0) / 1e9
+    mean_k = head["iterations"]["mean"]
+    step_roof = n * step_bytes(mean_k) * args.steps / (head["ms_per_step"] * args.steps / 1000.0) / 1e9
     out = {
         "metric": "particle-steps/s at <=1% density error (plain vs learned warm start), 1 GPU",
         "value": value,
@@ -158,7 +207,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["ms"] / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -168,13 +217,15 @@ def main():
                    "method": "PCISPH" if scene.params.method == 1 else "SPH",
                    "tolerance": tol, "min_iterations": scene.params.min_iterations,
                    "max_iterations": scene.params.max_iterations,
-                   "stall_iterations": scene.params.stall_iterations, "warm_start": "none"},
-        "iterations": {"mean": mean_k, "p50": float(np.percentile(its, 50)) if its.size else 0,
-                       "p99": float(np.percentile(its, 99)) if its.size else 0,
-                       "max": int(its.max()) if its.size else 0},
-        "qualified": nonconv == 0,
-        "nonconverged_steps": nonconv,
-        "worst_density_error": float(errs.max()) if errs.size else 0.0,
+                   "stall_iterations": scene.params.stall_iterations,
+                   "warm_start": "learned" if learned is not None else "none"},
+        "iterations": head["iterations"],
+        "qualified": head["qualified"],
+        "nonconverged_steps": head["nonconverged_steps"],
+        "worst_density_error": head["worst_density_error"],
+        "plain": plain,
+        "learned": learned,
+        "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned is not None else None,
         "roofline": roof,
         "step_roofline": {"achieved_GBps": step_roof, "frac": step_roof / HBM_PEAK_GBS,
                           "bytes_per_particle_step": step_bytes(mean_k)},

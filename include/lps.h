@@ -46,7 +46,8 @@ typedef enum lps_method { LPS_METHOD_SPH = 0, LPS_METHOD_PCISPH = 1 } lps_method
 
 typedef enum lps_warm_start {
     LPS_WARM_NONE = 0,    /* pciInit: p = 0, fp = 0 (SPHKernels.cl:661-672) */
-    LPS_WARM_LEARNED = 1  /* MLP-seeded pressure, falls back to NONE when it does not help */
+    LPS_WARM_LEARNED = 1, /* MLP-seeded pressure, falls back to NONE when it does not help */
+    LPS_WARM_LEARNED_ALWAYS = 2 /* MLP-seeded pressure on every step, no fallback (measurement) */
 } lps_warm_start;
 
 /* Fluid and solver parameters.  Field meanings follow the reference setters

@@ -20,6 +20,7 @@ METHOD_SPH = 0
 METHOD_PCISPH = 1
 WARM_NONE = 0
 WARM_LEARNED = 1
+WARM_LEARNED_ALWAYS = 2
 
 STATUS_NAMES = {
     0: "LPS_OK",

@@ -211,6 +211,7 @@ int check_params(const lps_params* p) {
     if (!(p->particle_radius > 0.f) || !(p->kernel_radius_factor > 0.f) || !(p->rest_density > 0.f))
         return LPS_ERR_INVALID;
     if (p->method == LPS_METHOD_PCISPH && (p->max_iterations < 1 || p->max_iterations > 10000)) return LPS_ERR_INVALID;
+    if (p->warm_start < LPS_WARM_NONE || p->warm_start > LPS_WARM_LEARNED_ALWAYS) return LPS_ERR_INVALID;
     return LPS_OK;
 }
 
@@ -387,7 +388,9 @@ struct StepResult {
 constexpr uint64_t WS_PROBE_PERIOD = 32;  // every 32nd step runs the other mode
 
 bool warm_ready(const lps_ctx* c) {
-    return c->params.method == LPS_METHOD_PCISPH && c->params.warm_start == LPS_WARM_LEARNED && c->mlp_loaded;
+    return c->params.method == LPS_METHOD_PCISPH &&
+           (c->params.warm_start == LPS_WARM_LEARNED || c->params.warm_start == LPS_WARM_LEARNED_ALWAYS) &&
+           c->mlp_loaded;
 }
 
 // Seed unless the averaged corrections per step with the seed exceed those
@@ -395,6 +398,7 @@ bool warm_ready(const lps_ctx* c) {
 // averages stay current.
 bool seed_this_step(const lps_ctx* c) {
     if (!warm_ready(c)) return false;
+    if (c->params.warm_start == LPS_WARM_LEARNED_ALWAYS) return true;
     if (c->ema_seeded < 0.0) return true;   // first measurement with the seed,
     if (c->ema_plain < 0.0) return false;   // then one without
     const bool probe = (c->ws_steps % WS_PROBE_PERIOD) == WS_PROBE_PERIOD - 1u;

@@ -60,7 +60,7 @@ __device__ __forceinline__ bf16x4 load4(const unsigned short* p) {
     return r;
 }
 
-struct MlpSmem {
+struct alignas(16) MlpSmem {
     unsigned short w1[HID * IN];    // [neuron][feature]
     unsigned short w2[HID * HID];   // [out][in]
     unsigned short w2t[HID * HID];  // [in][out]
@@ -190,11 +190,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_grad(MlpArgs m, const float* 
                                                         float inv_n, float over_weight, float* __restrict__ grad,
                                                         float* __restrict__ loss) {
     __shared__ MlpSmem S;
-    __shared__ float G[LPS_MLP_PARAMS];
-    __shared__ unsigned short T_[LPS_BLOCK / 64][HID * 16];   // dY2 [neuron][sample]
-    __shared__ unsigned short U_[LPS_BLOCK / 64][HID * 16];   // H1  [neuron][sample]
-    __shared__ unsigned short D_[LPS_BLOCK / 64][HID * 16];   // dY1 [neuron][sample]
-    __shared__ unsigned short V_[LPS_BLOCK / 64][IN * 16];    // X   [feature][sample]
+    // load4 reads 8 bytes at a time: every staging array is 16-byte aligned
+    __shared__ __align__(16) float G[LPS_MLP_PARAMS];
+    __shared__ __align__(16) unsigned short T_[LPS_BLOCK / 64][HID * 16];   // dY2 [neuron][sample]
+    __shared__ __align__(16) unsigned short U_[LPS_BLOCK / 64][HID * 16];   // H1  [neuron][sample]
+    __shared__ __align__(16) unsigned short D_[LPS_BLOCK / 64][HID * 16];   // dY1 [neuron][sample]
+    __shared__ __align__(16) unsigned short V_[LPS_BLOCK / 64][IN * 16];    // X   [feature][sample]
     __shared__ float s_loss;
     load_weights(S, m, true);
     for (uint32_t k = threadIdx.x; k < LPS_MLP_PARAMS; k += blockDim.x) G[k] = 0.f;

@@ -44,10 +44,16 @@ def load(path: str = WEIGHTS) -> dict:
 
 
 def save(path: str, w: dict, meta: Optional[dict] = None) -> None:
+    """Write the .npz (with training metadata) and, beside it, a .bin for C/C++
+    callers: little-endian float32 params[LPS_MLP_PARAMS], feature_scale[32],
+    target_scale."""
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez(path, params=np.asarray(w["params"], np.float32),
-             feature_scale=np.asarray(w["feature_scale"], np.float32),
-             target_scale=np.float32(w["target_scale"]), meta=json.dumps(meta or {}))
+    params = np.asarray(w["params"], np.float32)
+    fs = np.asarray(w["feature_scale"], np.float32)
+    np.savez(path, params=params, feature_scale=fs, target_scale=np.float32(w["target_scale"]),
+             meta=json.dumps(meta or {}))
+    raw = np.concatenate([params, fs, np.array([w["target_scale"]], np.float32)]).astype("<f4")
+    raw.tofile(os.path.splitext(path)[0] + ".bin")
 
 
 def apply(solver, w: Optional[dict] = None) -> None:
