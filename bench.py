@@ -58,16 +58,28 @@ def step_bytes(k: float, passes: int = 3) -> float:
     return 300.0 + 16.0 * passes + 108.0 * k
 
 
-def run_gpu(scene, steps: int, warmup: int, profile_window: int, weights=None):
-    import torch
+KERNELS = ("grid_setup", "keys", "radix_pass", "reorder", "cell_table", "density", "forces", "pressure_sph",
+           "pci_density", "mlp_seed", "pci_pressure", "integrate")
+
+
+def _solver(scene, weights):
     from liquidsimulation_amd.solver import solver_for
     from liquidsimulation_amd import warm_start
-
     s = solver_for(scene.params)
     if weights is not None:
         warm_start.apply(s, weights)
     s.set_colliders(scene.colliders)
     s.set_particles(scene.x, scene.v)
+    return s
+
+
+def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None):
+    """Timed run, then (profile) the same warm-up + steps again from the same
+    initial state with per-launch HIP events on the library stream: the run is
+    deterministic, so the per-kernel averages are those of the timed steps."""
+    import torch
+
+    s = _solver(scene, weights)
     if warmup:
         s.update(scene.dt, warmup)
     stream = torch.cuda.ExternalStream(s.stream())
@@ -92,22 +104,25 @@ def run_gpu(scene, steps: int, warmup: int, profile_window: int, weights=None):
               "potential_J": float(-m * np.sum(st["x"].astype(np.float64) @ g))}
     # dominant kernel: per-launch HIP events on the library stream over a window
     # of steps continuing from the timed state
+    s.close()
     kernels = {}
-    win = []
-    if profile_window > 0:
+    same = None
+    if profile:
+        s = _solver(scene, weights)
+        if warmup:
+            s.update(scene.dt, warmup)
         s.profile(True)
         s.profile_reset()
-        win = s.update(scene.dt, profile_window)
-        for name in ("grid_setup", "keys", "radix_pass", "reorder", "cell_table", "density", "forces",
-                     "pressure_sph", "pci_density", "mlp_seed", "pci_pressure", "integrate"):
+        rerun = s.update(scene.dt, steps)
+        for name in KERNELS:
             n, tot = s.profile_query(name)
             if n:
                 kernels[name] = {"launches": n, "total_ms": tot, "avg_us": 1000.0 * tot / n}
         s.profile(False)
-    win_its = np.array([x.iterations for x in win], np.int64)
-    s.close()
+        same = bool(np.array_equal(np.array([x.iterations for x in rerun], np.int64), its))
+        s.close()
     return {"ms": ms, "wall": wall, "its": its, "errs": errs, "conv": conv, "used": used, "kernels": kernels,
-            "win_its": win_its, "energy": energy}
+            "energy": energy, "rerun_identical": same}
 
 
 def mode_summary(res, scene, steps):
@@ -127,6 +142,7 @@ def mode_summary(res, scene, steps):
         "seeded_steps": int(res["used"].sum()),
         "end_energy": res["energy"],
         "kernels_avg_us": {k: v["avg_us"] for k, v in res["kernels"].items()},
+        "profiled_rerun_identical": res["rerun_identical"],
     }
 
 
@@ -159,7 +175,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="pcisph_dam_break_524k")
-    ap.add_argument("--profile-window", type=int, default=100)
+    ap.add_argument("--no-profile", action="store_true",
+                    help="skip the event-timed rerun that gives per-kernel averages")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-warm", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=20)
@@ -177,10 +194,10 @@ def main():
     if args.mode == "both" and scene.params.method == 1 and os.path.exists(warm_start.WEIGHTS):
         w = warm_start.load()
         sl = scene.with_params(warm_start=WARM_LEARNED)
-        learned = mode_summary(run_gpu(sl, args.steps, args.warmup, args.profile_window, weights=w), sl,
+        learned = mode_summary(run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w), sl,
                                args.steps)
     scene = scene.with_params(warm_start=WARM_NONE)
-    res = run_gpu(scene, args.steps, args.warmup, args.profile_window)
+    res = run_gpu(scene, args.steps, args.warmup, not args.no_profile)
     plain = mode_summary(res, scene, args.steps)
     n = scene.n
     head = learned if learned is not None else plain

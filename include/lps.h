@@ -115,10 +115,17 @@ int lps_set_params(lps_ctx* ctx, const lps_params* params);
 int lps_get_derived(lps_ctx* ctx, float* kernel_radius, float* division_step, float* mass,
                     float* tables /* 5 x LPS_TABLE_SIZE: W, W', W'' (poly6), spiky', visc'' */);
 int lps_get_delta(lps_ctx* ctx, float dt, float* delta);
+/* Lookup-index thresholds (no context, no device): thr[k], k = 1..1000, is
+ * the smallest float s >= 0 with min(trunc(sqrtf(s) / (h/1000)), 1000) >= k,
+ * thr[0] = 0 and thr[1001] = +inf, so the table index of a squared distance s
+ * is the number of k >= 1 with thr[k] <= s.  The stencils use this in place
+ * of the square root and division of SPHKernels.cl:237. */
+int lps_index_thresholds(const lps_params* params, float* thr /* [LPS_TABLE_SIZE + 1] */);
 
 /* Scene.  lps_set_particles replaces the upload of SPHSolver::initParallelBuffers
  * (SPHSolver.cpp:788-886); arrays are xyz-interleaved [3n]; vhalf and first may
- * be NULL (zero half velocity, first step set). */
+ * be NULL (zero half velocity, first step set).  1 <= n <= LPS_MAX_PARTICLES. */
+#define LPS_MAX_PARTICLES ((1u << 28) - 1u)
 int lps_set_particles(lps_ctx* ctx, const float* x, const float* v, const float* vhalf,
                       const uint8_t* first, uint32_t n);
 /* Readback of SPHSolver::onEndUpdate (SPHSolver.cpp:398-428), in sorted order.

@@ -186,6 +186,8 @@ SimConst make_const(const lps_ctx* c, float dt, float delta) {
     k.h = c->der.h;
     k.div = c->der.div;
     k.s_max = c->der.s_max;
+    k.s_relax = c->der.s_relax;
+    k.inv_div = c->der.inv_div;
     k.mass = c->der.mass;
     k.rest_density = c->params.rest_density;
     k.stiffness = c->params.stiffness;
@@ -216,9 +218,11 @@ int check_params(const lps_params* p) {
 }
 
 int upload_tables(lps_ctx* c) {
-    if (!c->tables) HIPC(hipMalloc((void**)&c->tables, sizeof(float) * 5 * LPS_TABLE_SIZE));
+    if (!c->tables) HIPC(hipMalloc((void**)&c->tables, sizeof(float) * LPS_TABLE_FLOATS));
     HIPC(hipMemcpyAsync(c->tables, c->der.tables, sizeof(float) * 5 * LPS_TABLE_SIZE, hipMemcpyHostToDevice,
                         c->stream));
+    HIPC(hipMemcpyAsync(c->tables + 5 * LPS_TABLE_SIZE, c->der.thr, sizeof(float) * LPS_THR_SIZE,
+                        hipMemcpyHostToDevice, c->stream));
     return LPS_OK;
 }
 
@@ -268,13 +272,13 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if (n <= c->cap_n && c->pos[0]) return LPS_OK;
     int st;
     for (int b = 0; b < 2; b++) {
-        if ((st = dalloc(&c->pos[b], n))) return st;
+        if ((st = dalloc(&c->pos[b], (size_t)n + LPS_STENCIL_PAD))) return st;
         if ((st = dalloc(&c->vel[b], n))) return st;
         if ((st = dalloc(&c->vh[b], n))) return st;
         if ((st = dalloc(&c->first[b], n))) return st;
         if ((st = dalloc(&c->id[b], n))) return st;
         if ((st = dalloc(&c->pprev[b], n))) return st;
-        if ((st = dalloc(&c->xpred[b], n))) return st;
+        if ((st = dalloc(&c->xpred[b], (size_t)n + LPS_STENCIL_PAD))) return st;
     }
     if ((st = dalloc(&c->keysA, n))) return st;
     if ((st = dalloc(&c->valsA, n))) return st;
@@ -285,7 +289,7 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if ((st = dalloc(&c->hist, (size_t)256 * lps_dev::radix_tiles(n)))) return st;
     if ((st = dalloc(&c->digit_total, 256))) return st;
     if ((st = dalloc(&c->bbox_part, (size_t)LPS_BBOX_PART * lps_dev::bbox_parts(n)))) return st;
-    if ((st = dalloc(&c->posrho, n))) return st;
+    if ((st = dalloc(&c->posrho, (size_t)n + LPS_STENCIL_PAD))) return st;
     if ((st = dalloc(&c->force, n))) return st;
     if ((st = dalloc(&c->fp, n))) return st;
     if ((st = dalloc(&c->press, n))) return st;
@@ -624,6 +628,16 @@ int lps_get_derived(lps_ctx* c, float* h, float* div, float* mass, float* tables
     return LPS_OK;
 }
 
+int lps_index_thresholds(const lps_params* p, float* thr) {
+    if (check_params(p) != LPS_OK || !thr) return LPS_ERR_INVALID;
+    Derived* d = new (std::nothrow) Derived();
+    if (!d) return LPS_ERR_OOM;
+    lps_compute_derived(p, d);
+    std::memcpy(thr, d->thr, sizeof(float) * LPS_THR_SIZE);
+    delete d;
+    return LPS_OK;
+}
+
 int lps_get_delta(lps_ctx* c, float dt, float* delta) {
     if (!c || !delta) return LPS_ERR_INVALID;
     *delta = lps_compute_delta(&c->params, &c->der, dt);
@@ -649,7 +663,7 @@ int lps_set_colliders(lps_ctx* c, const lps_box* boxes, uint32_t nbox, const lps
 
 int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* vhalf, const uint8_t* first,
                       uint32_t n) {
-    if (!c || !x || n == 0) return LPS_ERR_INVALID;
+    if (!c || !x || n == 0 || n > LPS_MAX_PARTICLES) return LPS_ERR_INVALID;
     int st = ensure_particles(c, n);
     if (st) return st;
     if ((st = ensure_cells(c, cell_bound(c, x, n)))) return st;

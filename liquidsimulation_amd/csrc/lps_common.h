@@ -54,6 +54,8 @@ struct SimConst {
     float h;            // kernel radius
     float div;          // table step h / 1000
     float s_max;        // largest float s with sqrtf(s) < h: d < h  <=>  d^2 <= s_max
+    float s_relax;      // s_max widened by 2^-18: bound for a candidate filter evaluated with FMA
+    float inv_div;      // 1 / div, for the first guess of a table index
     float mass;
     float rest_density;
     float stiffness;
@@ -82,10 +84,25 @@ static inline __host__ __device__ float o2f(uint32_t u) {
     return c.f;
 }
 
+// Position arrays scanned by the stencils carry this many padding entries:
+// the candidate filter reads whole groups of eight past a row's end.
+#define LPS_STENCIL_PAD 8
+
+// Device table block: the five lookup tables, then the index thresholds.
+#define LPS_THR_SIZE (LPS_TABLE_SIZE + 1)
+#define LPS_TABLE_FLOATS (5 * LPS_TABLE_SIZE + LPS_THR_SIZE)
+
 // Host-side derived quantities (lps_host.cpp).
 struct Derived {
-    float h, div, mass, s_max;
+    float h, div, mass, s_max, s_relax, inv_div;
     float tables[5][LPS_TABLE_SIZE];  // W, W', W'' (poly6), spiky', viscosity''
+    // thr[k] (k = 1..1000): smallest float s >= 0 whose lookup index
+    // min(trunc(sqrtf(s) / div), 1000) is >= k; thr[0] = 0, thr[1001] = +inf.
+    // The index of a squared distance s is then the number of k >= 1 with
+    // thr[k] <= s, so stencils need neither the square root nor the division
+    // to find it.
+    float thr[LPS_THR_SIZE];
 };
+void lps_index_thresholds(float h, float div, float* thr);
 void lps_compute_derived(const lps_params* p, Derived* d);
 float lps_compute_delta(const lps_params* p, const Derived* d, float dt);

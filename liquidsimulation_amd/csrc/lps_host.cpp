@@ -13,7 +13,9 @@
 // so they are produced here with the reference's float arithmetic: the kernel
 // coefficients are evaluated in double and rounded to float, powers of the
 // radius are accumulated in float.
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 #include "lps_common.h"
@@ -81,6 +83,52 @@ void lps_compute_derived(const lps_params* p, Derived* d) {
     while (std::sqrt(s) >= r) s = std::nextafter(s, 0.f);
     while (std::sqrt(std::nextafter(s, INFINITY)) < r) s = std::nextafter(s, INFINITY);
     d->s_max = s;
+    // a squared distance evaluated with fused multiply-adds differs from the
+    // unfused one by a few units in the last place; 2^-18 covers that
+    d->s_relax = std::nextafter(s * (1.f + 1.f / 262144.f), INFINITY);
+    d->inv_div = 1.f / step;
+    lps_index_thresholds(r, step, d->thr);
+}
+
+// The lookup index of the OpenCL kernels, min(trunc(d / step), 1000) with
+// d = sqrtf(s) (e.g. SPHKernels.cl:237), as a function of the squared distance s.
+static uint32_t lookup_index(float s, float step) {
+    const float q = std::sqrt(s) / step;
+    if (!(q < (float)LPS_TABLE_SIZE)) return LPS_TABLE_SIZE - 1;
+    return std::min((uint32_t)std::trunc(q), (uint32_t)(LPS_TABLE_SIZE - 1));
+}
+
+void lps_index_thresholds(float h, float div, float* thr) {
+    // the index is monotone in s (sqrtf and the division round to nearest and
+    // are monotone), and non-negative floats order like their bit patterns:
+    // binary search per k over the bit patterns up to just beyond h^2
+    uint32_t hi_bits;
+    const float s_hi = std::nextafter(h * h * 1.01f, INFINITY);
+    std::memcpy(&hi_bits, &s_hi, sizeof(float));
+    auto at = [](uint32_t b) {
+        float f;
+        std::memcpy(&f, &b, sizeof(float));
+        return f;
+    };
+    thr[0] = 0.f;
+    for (uint32_t k = 1; k < (uint32_t)LPS_TABLE_SIZE; k++) {
+        if (lookup_index(s_hi, div) < k) {
+            thr[k] = INFINITY;
+            continue;
+        }
+        uint32_t lo = 0, hi = hi_bits;  // index(at(lo)) < k <= index(at(hi)) once lo is excluded
+        if (lookup_index(0.f, div) >= k) {
+            thr[k] = 0.f;
+            continue;
+        }
+        while (hi - lo > 1u) {
+            const uint32_t mid = lo + (hi - lo) / 2u;
+            if (lookup_index(at(mid), div) >= k) hi = mid;
+            else lo = mid;
+        }
+        thr[k] = at(hi);
+    }
+    thr[LPS_TABLE_SIZE] = INFINITY;
 }
 
 float lps_compute_delta(const lps_params* p, const Derived* d, float dt) {

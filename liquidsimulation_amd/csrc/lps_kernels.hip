@@ -60,6 +60,12 @@ __device__ __forceinline__ void load_table(float* dst, const float* src) {
     for (uint32_t k = threadIdx.x; k < LPS_TABLE_SIZE; k += blockDim.x) dst[k] = src[k];
 }
 
+// index thresholds, stored after the five tables
+__device__ __forceinline__ void load_thresholds(float* dst, const float* tables) {
+    const float* src = tables + 5 * LPS_TABLE_SIZE;
+    for (uint32_t k = threadIdx.x; k < LPS_THR_SIZE; k += blockDim.x) dst[k] = src[k];
+}
+
 // ---------------------------------------------------- grid traversal --
 struct GridView {
     float off0, off1, off2, gs;
@@ -104,24 +110,31 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
 }
 
 // The same stencil in two passes per window of up to 32 candidates of a row:
-// pass 1 tests d^2 <= s_max for every candidate (four independent loads in
-// flight) and records a per-lane accept mask; pass 2 visits the accepted
-// candidates in ascending order.  The accepted set and the visiting order are
-// those of for_neighbors, so sums are bit-identical, but the expensive pair
-// work (IEEE sqrt and divisions) no longer runs once per candidate for the
-// whole wave, only once per accepted pair of the busiest lane.
-template <class Acc, class F>
+// pass 1 runs a cheap filter over every candidate (eight independent loads in
+// flight; squared distance with fused multiply-adds against s_relax, which
+// bounds its rounding difference to the exact form) and records a per-lane
+// accept mask; pass 2 visits the kept candidates in ascending order, four
+// loads at a time (position and pre(j)), forms the exact squared distance s2
+// (the kernels' unfused expression) and calls fn(j, x_j, s2, pre(j)) when
+// s2 <= s_max, i.e. when d = sqrtf(s2) < h.  Accepted set and visiting order
+// are those of for_neighbors, so sums are bit-identical.  Candidates are
+// addressed by 32-bit byte offsets from xs (particle counts stay below 2^28);
+// pass 1 reads up to 7 entries past a row's end, so xs carries
+// LPS_STENCIL_PAD entries of padding (masked out, never used).
+template <class Acc, class Pre, class F>
 __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const uint32_t* __restrict__ lb, float ax,
                                                      float ay, float az, const float4* __restrict__ xs, float px,
-                                                     float py, float pz, float s_max, Acc&& accept_extra, F&& fn) {
+                                                     float py, float pz, float s_max, float s_relax,
+                                                     Acc&& accept_extra, Pre&& pre, F&& fn) {
     const int xg = (int)truncf((ax + G.off0) / G.gs);
     const int yg = (int)truncf((ay + G.off1) / G.gs);
     const int zg = (int)truncf((az + G.off2) / G.gs);
     const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
-    auto hit = [&](const float4& q) {
+    const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
+    auto at = [&](uint32_t j) { return *reinterpret_cast<const float4*>(xb + (j << 4)); };
+    auto keep = [&](const float4& q) {
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
-        const float s2 = dx * dx + dy * dy + dz * dz;
-        return s2 <= s_max;
+        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) <= s_relax;
     };
     for (int z = zg - 1; z <= zg + 1; z++) {
         if (z < 0 || z >= G.gz) continue;
@@ -133,24 +146,61 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
             for (uint32_t w0 = b; w0 < e; w0 += 32u) {
                 const uint32_t n = min(32u, e - w0);
                 uint32_t mask = 0u;
-                uint32_t u = 0;
-                for (; u + 4u <= n; u += 4u) {
-                    const float4 q0 = xs[w0 + u], q1 = xs[w0 + u + 1u], q2 = xs[w0 + u + 2u], q3 = xs[w0 + u + 3u];
-                    mask |= (hit(q0) ? 1u : 0u) << u;
-                    mask |= (hit(q1) ? 2u : 0u) << u;
-                    mask |= (hit(q2) ? 4u : 0u) << u;
-                    mask |= (hit(q3) ? 8u : 0u) << u;
+                for (uint32_t u = 0; u < n; u += 8u) {
+                    const char* p8 = xb + ((w0 + u) << 4);
+                    float4 q[8];
+#pragma unroll
+                    for (int t = 0; t < 8; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
+                    uint32_t m8 = 0u;
+#pragma unroll
+                    for (int t = 0; t < 8; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
+                    mask |= m8 << u;
                 }
-                for (; u < n; u++) mask |= (hit(xs[w0 + u]) ? 1u : 0u) << u;
+                mask &= n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u;
                 while (mask) {
-                    const uint32_t k = (uint32_t)__ffs((int)mask) - 1u;
+                    uint32_t kk[4];
+                    uint32_t cnt = 1u;
+                    kk[0] = (uint32_t)__ffs((int)mask) - 1u;
                     mask &= mask - 1u;
-                    const uint32_t j = w0 + k;
-                    if (accept_extra(j)) fn(j);
+#pragma unroll
+                    for (int t = 1; t < 4; t++) {
+                        kk[t] = kk[0];
+                        if (mask) {
+                            kk[t] = (uint32_t)__ffs((int)mask) - 1u;
+                            mask &= mask - 1u;
+                            cnt++;
+                        }
+                    }
+                    float4 q[4];
+                    decltype(pre(0u)) pv[4];
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        q[t] = at(w0 + kk[t]);
+                        pv[t] = pre(w0 + kk[t]);
+                    }
+#pragma unroll
+                    for (int t = 0; t < 4; t++) {
+                        if ((uint32_t)t < cnt) {
+                            const uint32_t j = w0 + kk[t];
+                            const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
+                            const float s2 = dx * dx + dy * dy + dz * dz;
+                            if (s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t]);
+                        }
+                    }
                 }
             }
         }
     }
+}
+
+// Lookup index min(trunc(sqrtf(s2) / div), 1000) of a squared distance
+// s2 <= s_max from the threshold table (see Derived::thr): the hardware
+// square root and 1/div give the quotient to within 5e-4; one below that,
+// truncated, is the exact index or one less, and the threshold decides.
+__device__ __forceinline__ uint32_t table_index(float s2, const float* __restrict__ thr, float inv_div) {
+    const float qa = __builtin_amdgcn_sqrtf(s2) * inv_div - 1e-3f;
+    const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 1));
+    return g + (s2 >= thr[g + 1u] ? 1u : 0u);
 }
 
 // ---------------------------------------------------------- colliders --
@@ -686,21 +736,18 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
-    __shared__ float sW[LPS_TABLE_SIZE];
+    __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sW, tables);
+    load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     if (i >= c.n) return;
     const GridView G = grid_view(g);
     const float4 xi = pos[i];
     float sum = 0.f;
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max,
-                         [](uint32_t) { return true; }, [&](uint32_t j) {
-        const float4 xj = pos[j];
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-        sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-    });
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
+                         [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+                         [&](uint32_t, const float4&, float s2, int) { sum += sW[table_index(s2, sT, c.inv_div)]; });
     const float rho = c.mass * sum;
     float p = c.stiffness * (rho - c.rest_density);
     if (p < 0.f) p *= c.npf;
@@ -722,10 +769,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
-    __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE];
+    __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sW1, tables + 1 * LPS_TABLE_SIZE);
     load_table(sW2, tables + 2 * LPS_TABLE_SIZE);
     load_table(sV2, tables + 4 * LPS_TABLE_SIZE);
+    load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     if (i >= c.n) return;
@@ -736,12 +784,10 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
     float nb = 0.f, dv = 0.f;  // warm-start features: neighbour count, velocity divergence
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max,
-                         [](uint32_t) { return true; }, [&](uint32_t j) {
-        const float4 xj = posrho[j];
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-        const uint32_t idx = min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1));
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
+                         [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
+                         [&](uint32_t j, const float4& xj, float s2, const float4& vj) {
+        const uint32_t idx = table_index(s2, sT, c.inv_div);
         const float rj = xj.w;
         const float w1 = sW1[idx];
         snx += (xi.x - xj.x) * w1 / rj;
@@ -749,7 +795,6 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
         snz += (xi.z - xj.z) * w1 / rj;
         lc += sW2[idx] / rj;
         if (i != j) {
-            const float4 vj = vel[j];
             const float wv = sV2[idx];
             vfx += (vj.x - vi.x) * wv / rj;
             vfy += (vj.y - vi.y) * wv / rj;
@@ -807,8 +852,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
-    __shared__ float sP1[LPS_TABLE_SIZE];
+    __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
+    load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     if (i >= c.n) return;
@@ -816,20 +862,17 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const float4 xi = posrho[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max,
-                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) {
-        const float4 xj = posrho[j];
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        {
-            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-            float ux, uy, uz;
-            pair_dir(xi, xj, d, i, j, ux, uy, uz);
-            const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-            const float sc = tf + q[j];
-            pfx += ux * sc * w;
-            pfy += uy * sc * w;
-            pfz += uz * sc * w;
-        }
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
+                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+                         [&](uint32_t j, const float4& xj, float s2, float qj) {
+        const float d = sqrtf(s2);
+        float ux, uy, uz;
+        pair_dir(xi, xj, d, i, j, ux, uy, uz);
+        const float w = sP1[table_index(s2, sT, c.inv_div)];
+        const float sc = tf + qj;
+        pfx += ux * sc * w;
+        pfy += uy * sc * w;
+        pfz += uz * sc * w;
     });
     float4 f = force[i];
     f.x += (-pfx) * c.mass * xi.w;
@@ -867,10 +910,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         return;
     }
     const uint32_t lb = xcd_block(nlogical);
-    __shared__ float sW[LPS_TABLE_SIZE];
+    __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
     __shared__ uint32_t s_last;
     load_table(sW, tables);
+    load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = (lb == UINT_MAX) ? UINT_MAX : lb * LPS_BLOCK + threadIdx.x;
     uint32_t ebits = 0;
@@ -879,13 +923,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         const float4 xa = pos[i];
         const float4 xi = xpred[i];
         float sum = 0.f;
-        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max,
-                             [](uint32_t) { return true; }, [&](uint32_t j) {
-            const float4 xj = xpred[j];
-            const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
-            sum += sW[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-        });
+        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
+                             [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+                             [&](uint32_t, const float4&, float s2, int) { sum += sW[table_index(s2, sT, c.inv_div)]; });
         const float rp = c.mass * sum;
         float e = rp - c.rest_density;
         float dp = c.delta * e;
@@ -967,8 +1007,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     if (step_dead(g) || (g->stop_iter != 0u && g->stop_iter < k)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
-    __shared__ float sP1[LPS_TABLE_SIZE];
+    __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
+    load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     if (i >= c.n) return;
@@ -977,15 +1018,14 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4 xi = xpred[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max,
-                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) {
-        const float4 xj = xpred[j];
-        const float dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
-        const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
+                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+                         [&](uint32_t j, const float4& xj, float s2, float qj) {
+        const float d = sqrtf(s2);
         float ux, uy, uz;
         pair_dir(xi, xj, d, i, j, ux, uy, uz);
-        const float w = sP1[min((uint32_t)truncf(d / c.div), (uint32_t)(LPS_TABLE_SIZE - 1))];
-        const float sc = tf + q[j];
+        const float w = sP1[table_index(s2, sT, c.inv_div)];
+        const float sc = tf + qj;
         pfx += ux * sc * w;
         pfy += uy * sc * w;
         pfz += uz * sc * w;

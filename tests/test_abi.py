@@ -6,6 +6,7 @@ import os
 import subprocess
 import tempfile
 
+import numpy as np
 import pytest
 
 from liquidsimulation_amd import _lib
@@ -122,3 +123,37 @@ def test_solver_fails_loudly_without_library(monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/liblps.so")
     with pytest.raises(_lib.LibraryMissing):
         _lib.lib()
+
+
+def test_index_thresholds_reproduce_sqrt_and_division():
+    """The stencils find the table index min(trunc(sqrtf(s)/step), 1000) of a
+    squared distance s from thresholds instead of a square root and a
+    division.  For every index boundary the threshold and its float
+    predecessor fall on either side; random squared distances (below h^2 and
+    near the boundaries) get the index the IEEE square root and division give."""
+    L = _lib.lib()
+    for r_p, factor in ((0.00390625, 3.2), (0.0125, 3.2), (0.0025, 3.2), (0.017, 2.69)):
+        p = Params(particle_radius=r_p, kernel_radius_factor=factor).to_c()
+        thr = np.zeros(1002, np.float32)
+        assert L.lps_index_thresholds(C.byref(p), thr) == 0
+        h = np.float32(np.float32(factor) * np.float32(r_p))
+        step = np.float32(h / np.float32(1000))
+
+        def index(s):
+            s = np.asarray(s, np.float32)
+            return np.minimum(np.trunc(np.sqrt(s) / step), 1000).astype(np.int64)
+
+        assert thr[0] == 0 and np.isinf(thr[1001])
+        k = np.arange(1, 1001)
+        finite = np.isfinite(thr[1:1001])
+        assert finite[:999].all()  # every index below 1000 is reachable under h^2
+        t = thr[1:1001][finite]
+        kk = k[finite]
+        assert np.all(index(t) >= kk)
+        assert np.all(index(np.nextafter(t, np.float32(0))) < kk)
+        assert np.all(np.diff(thr[1:1001][finite]) > 0)
+        rng = np.random.default_rng(int(r_p * 1e6))
+        s = np.concatenate([rng.uniform(0, 1, 200000).astype(np.float32) * h * h,
+                            np.nextafter(t, np.float32(0)), t, np.nextafter(t, np.float32(1))]).astype(np.float32)
+        via = np.searchsorted(thr[1:1001], s, side="right")  # number of k >= 1 with thr[k] <= s
+        np.testing.assert_array_equal(via, index(s))
