@@ -169,6 +169,35 @@ def cpu_baseline(scene, warm: int, timed: int):
     }
 
 
+def copy_calibration():
+    """A device-to-device copy of known size: rocprofv3's FETCH_SIZE and
+    WRITE_SIZE read on it give the factors that turn those counters into bytes."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 4, "copies": 5}), flush=True)
+
+
+TRAFFIC = os.path.join(ROOT, "profiles", "pcisph_dam_break_524k_traffic.json")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        return k["fetch_bytes"] + k["write_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -181,7 +210,12 @@ def main():
     ap.add_argument("--cpu-warm", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--mode", choices=("both", "plain"), default="both")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy five times (scales the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the SPH step has no multi-GPU path); "
               "run with --gpus 1", file=sys.stderr)
@@ -213,7 +247,7 @@ def main():
         if per:
             achieved = per * n / avg_s / 1e9
             roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom), "kernel": dom,
                     "algo_bytes_per_launch": per * n, "avg_launch_us": kernels[dom]["avg_us"]}
     mean_k = head["iterations"]["mean"]
     step_roof = n * step_bytes(mean_k) * args.steps / (head["ms_per_step"] * args.steps / 1000.0) / 1e9

@@ -757,11 +757,19 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
                 o.converged = r.converged;
             }
         }
+        c->steps = c->steps + batch;
+        c->have_step = true;
+        if (pci && last.ok) {
+            // the correction loop ended normally, which the host has already
+            // seen; no synchronisation before the next step
+            completed += 1;
+            retry_plain = false;
+            if (warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
+            continue;
+        }
         HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
         completed = c->h_grid->steps_ok - base;
-        c->steps = c->steps + batch;
-        c->have_step = true;
         const uint32_t status = c->h_grid->status;
         if (status == LPS_ERR_GRID_OVERFLOW) {
             const uint32_t need = c->h_grid->cells_needed;
@@ -784,7 +792,14 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
             if (stats && completed < nsteps) stats[completed].status = (int32_t)status;
             return (int)status;
         }
-        if (warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
+        if (pci && warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
+    }
+    if (pci) {
+        // one check per call: every step must have completed on the device
+        HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
+        HIPC(hipStreamSynchronize(c->stream));
+        if (c->h_grid->status != 0u) return (int)c->h_grid->status;
+        if (c->h_grid->steps_ok - base != nsteps) return LPS_ERR_HIP;
     }
     if (c->prof) prof_collect(c);
     return LPS_OK;
