@@ -137,13 +137,33 @@ int lps_get_count(lps_ctx* ctx, uint32_t* n);
 int lps_set_colliders(lps_ctx* ctx, const lps_box* boxes, uint32_t nbox, const lps_sphere* spheres,
                       uint32_t nsphere, const lps_plane* planes, uint32_t nplane);
 
+/* Ensembles (no counterpart in the reference): one context can hold up to
+ * LPS_MAX_SCENES independent scenes sharing the fluid parameters, each with
+ * its own gravity and colliders, stepped together by lps_step (one launch per
+ * phase for all scenes, one PCISPH loop rule per scene; a scene whose loop
+ * has stopped skips the remaining corrections).  Every scene's results equal
+ * those of the scene run alone.  Arrays are the scenes' particles
+ * concatenated in scene order; gravity is [3 * nscenes] or NULL (the
+ * parameters' gravity for all).  lps_set_particles is the single-scene case;
+ * lps_set_colliders gives every scene the same colliders. */
+#define LPS_MAX_SCENES 256
+int lps_set_scenes(lps_ctx* ctx, uint32_t nscenes, const uint32_t* counts, const float* gravity,
+                   const float* x, const float* v, const float* vhalf, const uint8_t* first);
+int lps_set_scene_colliders(lps_ctx* ctx, uint32_t scene, const lps_box* boxes, uint32_t nbox,
+                            const lps_sphere* spheres, uint32_t nsphere, const lps_plane* planes, uint32_t nplane);
+int lps_get_scene_count(lps_ctx* ctx, uint32_t* nscenes);
+/* Per-scene statistics of the last step (iterations, final error ratio, converged). */
+int lps_get_scene_stats(lps_ctx* ctx, lps_step_stats* stats /* [nscenes] */, uint32_t nscenes);
+
 /* One or more steps of PhysicSolver::update (PhysicSolver.cpp:12-21) on the device.
  * stats may be NULL or hold nsteps entries.  Returns the first failing status. */
 int lps_step(lps_ctx* ctx, float dt, uint32_t nsteps, lps_step_stats* stats);
 
 /* Parity hooks: the grid and fields of the last step (sorted order).
  * keys/perm [n]; cell_start/cell_end [cap] follow buildCellList's rule
- * (SPHKernels.cl:148-186, end rule :230).  Any pointer may be NULL. */
+ * (SPHKernels.cl:148-186, end rule :230).  Any pointer may be NULL.  With
+ * several scenes, info describes scene 0 and keys/cells are the global
+ * ones (scene s's cells follow scene s-1's and its one empty-slot cell). */
 int lps_get_grid(lps_ctx* ctx, lps_grid_info* info, uint32_t* keys, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* cell_end, uint32_t cap);
 int lps_get_fields(lps_ctx* ctx, float* rho, float* pressure, float* force /* [3n] */,

@@ -52,6 +52,10 @@ def lib():
     L.lps_get_particles.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32]
     L.lps_get_count.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.lps_set_colliders.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32]
+    L.lps_set_scenes.argtypes = [vp, C.c_uint32, _u32p, vp, _f32p, vp, vp, vp]
+    L.lps_set_scene_colliders.argtypes = [vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32]
+    L.lps_get_scene_count.argtypes = [vp, C.POINTER(C.c_uint32)]
+    L.lps_get_scene_stats.argtypes = [vp, vp, C.c_uint32]
     L.lps_step.argtypes = [vp, C.c_float, C.c_uint32, vp]
     L.lps_get_grid.argtypes = [vp, C.POINTER(CGridInfo), vp, vp, vp, vp, C.c_uint32]
     L.lps_get_fields.argtypes = [vp, vp, vp, vp, vp]

@@ -39,8 +39,9 @@ struct lps_ctx {
     hipStream_t stream = nullptr;
     int device = 0;
 
-    uint32_t n = 0;         // particles
-    uint32_t cap_n = 0;     // allocated particle capacity
+    uint32_t n = 0;         // particle slots: all scenes, each padded to whole blocks
+    uint32_t n_real = 0;    // particles
+    uint32_t cap_n = 0;     // allocated slot capacity
     uint32_t cell_cap = 0;  // allocated cell-table entries (plus one sentinel)
     uint32_t passes = 0;    // radix passes for cell_cap
     int cur = 0;            // which state buffer holds the sorted state
@@ -64,9 +65,18 @@ struct lps_ctx {
     lps_box* boxes = nullptr;
     lps_sphere* spheres = nullptr;
     lps_plane* planes = nullptr;
-    uint32_t nbox = 0, nsphere = 0, nplane = 0;
-    std::vector<lps_box> h_boxes;
     DevGrid* grid = nullptr;
+
+    // scenes (one for the single-scene API): layout, gravity, colliders
+    std::vector<SceneInfo> h_sinfo{SceneInfo{}};
+    std::vector<std::vector<lps_box>> s_boxes{{}};
+    std::vector<std::vector<lps_sphere>> s_spheres{{}};
+    std::vector<std::vector<lps_plane>> s_planes{{}};
+    std::vector<float> s_gravity;          // [3 * nscenes] when set per scene, else empty
+    SceneInfo* d_sinfo = nullptr;
+    SceneGrid* d_sgrid = nullptr;
+    SceneGrid* h_sgrid = nullptr;          // pinned copy for per-scene statistics
+    uint8_t* d_bscene = nullptr;           // scene of every particle block
 
     // host
     uint32_t* flags = nullptr;  // pinned, coherent: 4 words per iteration slot
@@ -200,9 +210,9 @@ SimConst make_const(const lps_ctx* c, float dt, float delta) {
     k.dt = dt;
     k.delta = delta;
     k.n = c->n;
-    k.nbox = c->nbox;
-    k.nsphere = c->nsphere;
-    k.nplane = c->nplane;
+    k.nbox = 0;     // per scene (SceneInfo)
+    k.nsphere = 0;
+    k.nplane = 0;
     k.cell_cap = c->cell_cap;
     return k;
 }
@@ -226,10 +236,10 @@ int upload_tables(lps_ctx* c) {
     return LPS_OK;
 }
 
-// Cell-count bound: particles end every step inside each BOUNDARY box shrunk
-// by the particle radius, so that box bounds the grid; the first step's grid
-// comes from the initial positions.
-uint32_t cell_bound(const lps_ctx* c, const float* x, uint32_t n) {
+// Cell-count bound of one scene: particles end every step inside each
+// BOUNDARY box shrunk by the particle radius, so that box bounds the grid; the
+// first step's grid comes from the initial positions.
+uint64_t scene_cell_bound(const lps_ctx* c, const float* x, uint32_t n, const std::vector<lps_box>& boxes) {
     const float h = c->der.h;
     uint64_t init = 1, bound = 0;
     float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -244,15 +254,14 @@ uint32_t cell_bound(const lps_ctx* c, const float* x, uint32_t n) {
         init *= (uint64_t)std::min(ex, 4.0e9) + 2;
     }
     init = std::min<uint64_t>(init, 0x7FFFFFF0ull);
-    for (const lps_box& b : c->h_boxes) {
+    for (const lps_box& b : boxes) {
         if (!b.boundary) continue;
         uint64_t v = 1;
         for (int a = 0; a < 3; a++) v *= (uint64_t)(2.f * std::max(b.half[a] - c->params.particle_radius, 0.f) / h) + 2;
         bound = bound ? std::min(bound, v) : v;
     }
     uint64_t cap = bound ? std::max(bound, init) : init * 8;
-    cap += cap / 8 + 64;
-    return (uint32_t)std::min<uint64_t>(cap, 0x7FFFFFF0ull);
+    return cap + cap / 8 + 64;
 }
 
 int ensure_cells(lps_ctx* c, uint32_t cap) {
@@ -313,14 +322,22 @@ void free_all(lps_ctx* c) {
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
     dfree(c->d_mlp); dfree(c->feat); dfree(c->seed); dfree(c->coll_x); dfree(c->coll_t);
     dfree(c->d_grad); dfree(c->d_m1); dfree(c->d_m2); dfree(c->d_loss); dfree(c->d_stats);
+    dfree(c->d_sinfo); dfree(c->d_sgrid); dfree(c->d_bscene);
+    if (c->h_sgrid) (void)hipHostFree(c->h_sgrid);
+    c->h_sgrid = nullptr;
     if (c->flags) (void)hipHostFree(c->flags);
     if (c->h_grid) (void)hipHostFree(c->h_grid);
     c->flags = nullptr;
     c->h_grid = nullptr;
 }
 
+lps_dev::Scenes scenes(const lps_ctx* c) {
+    return lps_dev::Scenes{c->d_sinfo, c->d_sgrid, c->d_bscene, (uint32_t)c->h_sinfo.size()};
+}
+
 lps_dev::PhysArgs phys_args(lps_ctx* c, const SimConst& k) {
     lps_dev::PhysArgs a;
+    a.sc = scenes(c);
     const int s = c->cur;
     a.pos = c->pos[s];
     a.vel = c->vel[s];
@@ -345,11 +362,11 @@ void enqueue_grid(lps_ctx* c) {
     const uint32_t n = c->n;
     {
         Timed t(c, "grid_setup");
-        lps_dev::launch_grid_setup(s, c->grid, c->bbox_part, n, c->der.h, c->cell_cap);
+        lps_dev::launch_grid_setup(s, c->grid, scenes(c), c->bbox_part, c->der.h, c->cell_cap);
     }
     {
         Timed t(c, "keys");
-        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, c->keysA, c->valsA);
+        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, scenes(c), c->keysA, c->valsA);
     }
     for (uint32_t p = 0; p < c->passes; p++) {
         Timed t(c, "radix_pass");
@@ -441,7 +458,7 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     lps_dev::PhysArgs a = phys_args(c, k);
     {
         Timed t(c, "density");
-        lps_dev::launch_density(s, a.pos, c->grid, c->cell_lb, k, c->tables, c->posrho, c->press, c->q);
+        lps_dev::launch_density(s, a, c->press, c->q);
     }
     if (!pci) {
         {
@@ -481,7 +498,7 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
             Timed t(c, "mlp_seed");
             lps_dev::launch_mlp_seed(s, mlp_args(c), c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
                                      delta * c->params.rest_density,
-                                     c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid);
+                                     c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c));
         }
         {
             Timed t(c, "pci_pressure");
@@ -510,12 +527,16 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     }
     if (feat_on && res.ok && c->coll_stride > 0 && c->coll_rows < c->coll_cap) {
         // training rows: every stride-th particle's features and final pressure
-        const uint32_t n = c->n, off = c->coll_offset % c->coll_stride;
-        uint64_t count = n > off ? ((uint64_t)(n - off) + c->coll_stride - 1u) / c->coll_stride : 0u;
-        count = std::min<uint64_t>(count, c->coll_cap - c->coll_rows);
-        lps_dev::launch_collect(s, c->feat, c->press, delta * c->params.rest_density, n, c->coll_stride, off,
-                                (uint32_t)count, c->coll_x, c->coll_t, c->coll_rows);
-        c->coll_rows += count;
+        const uint32_t off = c->coll_offset % c->coll_stride;
+        for (const SceneInfo& si : c->h_sinfo) {
+            const uint32_t n = si.count;
+            uint64_t count = n > off ? ((uint64_t)(n - off) + c->coll_stride - 1u) / c->coll_stride : 0u;
+            count = std::min<uint64_t>(count, c->coll_cap - c->coll_rows);
+            lps_dev::launch_collect(s, c->feat + (size_t)si.slot0 * LPS_MLP_IN, c->press + si.slot0,
+                                    delta * c->params.rest_density, n, c->coll_stride, off, (uint32_t)count,
+                                    c->coll_x, c->coll_t, c->coll_rows);
+            c->coll_rows += count;
+        }
         c->coll_offset = (off + 1u) % c->coll_stride;
     }
     {
@@ -526,6 +547,53 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     c->feat_valid = feat_on;
     c->seed_valid = seeded;
     return res;
+}
+
+// Concatenate the scenes' colliders into the device arrays and upload the
+// scene constants (collider ranges, gravity).
+int upload_scenes(lps_ctx* c) {
+    const size_t S = c->h_sinfo.size();
+    std::vector<lps_box> boxes;
+    std::vector<lps_sphere> spheres;
+    std::vector<lps_plane> planes;
+    for (size_t k = 0; k < S; k++) {
+        SceneInfo& si = c->h_sinfo[k];
+        si.box0 = (uint32_t)boxes.size();
+        si.nbox = (uint32_t)c->s_boxes[k].size();
+        si.sphere0 = (uint32_t)spheres.size();
+        si.nsphere = (uint32_t)c->s_spheres[k].size();
+        si.plane0 = (uint32_t)planes.size();
+        si.nplane = (uint32_t)c->s_planes[k].size();
+        boxes.insert(boxes.end(), c->s_boxes[k].begin(), c->s_boxes[k].end());
+        spheres.insert(spheres.end(), c->s_spheres[k].begin(), c->s_spheres[k].end());
+        planes.insert(planes.end(), c->s_planes[k].begin(), c->s_planes[k].end());
+        for (int a = 0; a < 3; a++)
+            si.gravity[a] = c->s_gravity.empty() ? c->params.gravity[a] : c->s_gravity[3 * k + a];
+    }
+    if (boxes.size() > LPS_MAX_COLLIDERS || spheres.size() > LPS_MAX_COLLIDERS || planes.size() > LPS_MAX_COLLIDERS)
+        return LPS_ERR_INVALID;
+    if (!boxes.empty())
+        HIPC(hipMemcpyAsync(c->boxes, boxes.data(), sizeof(lps_box) * boxes.size(), hipMemcpyHostToDevice, c->stream));
+    if (!spheres.empty())
+        HIPC(hipMemcpyAsync(c->spheres, spheres.data(), sizeof(lps_sphere) * spheres.size(), hipMemcpyHostToDevice,
+                            c->stream));
+    if (!planes.empty())
+        HIPC(hipMemcpyAsync(c->planes, planes.data(), sizeof(lps_plane) * planes.size(), hipMemcpyHostToDevice,
+                            c->stream));
+    if (c->d_sinfo)
+        HIPC(hipMemcpyAsync(c->d_sinfo, c->h_sinfo.data(), sizeof(SceneInfo) * S, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    return LPS_OK;
+}
+
+// Real particles of every scene, in slot order, from a full slot array.
+template <class T>
+void compact_slots(const lps_ctx* c, const std::vector<T>& full, T* out, size_t width) {
+    size_t r = 0;
+    for (const SceneInfo& si : c->h_sinfo) {
+        std::memcpy(out + r * width, full.data() + (size_t)si.slot0 * width, sizeof(T) * si.count * width);
+        r += si.count;
+    }
 }
 
 }  // namespace
@@ -616,7 +684,8 @@ int lps_set_params(lps_ctx* c, const lps_params* p) {
     if (st) return st;
     c->params = *p;
     lps_compute_derived(&c->params, &c->der);
-    return upload_tables(c);
+    st = upload_tables(c);
+    return st ? st : upload_scenes(c);
 }
 
 int lps_get_derived(lps_ctx* c, float* h, float* div, float* mass, float* tables) {
@@ -649,28 +718,94 @@ int lps_set_colliders(lps_ctx* c, const lps_box* boxes, uint32_t nbox, const lps
     if (!c || nbox > LPS_MAX_COLLIDERS || nsphere > LPS_MAX_COLLIDERS || nplane > LPS_MAX_COLLIDERS)
         return LPS_ERR_INVALID;
     if ((nbox && !boxes) || (nsphere && !spheres) || (nplane && !planes)) return LPS_ERR_INVALID;
-    if (nbox) HIPC(hipMemcpyAsync(c->boxes, boxes, sizeof(lps_box) * nbox, hipMemcpyHostToDevice, c->stream));
-    if (nsphere)
-        HIPC(hipMemcpyAsync(c->spheres, spheres, sizeof(lps_sphere) * nsphere, hipMemcpyHostToDevice, c->stream));
-    if (nplane) HIPC(hipMemcpyAsync(c->planes, planes, sizeof(lps_plane) * nplane, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    c->nbox = nbox;
-    c->nsphere = nsphere;
-    c->nplane = nplane;
-    c->h_boxes.assign(boxes, boxes + nbox);
-    return LPS_OK;
+    const size_t S = c->h_sinfo.size();
+    if (S * (nbox + nsphere + nplane) > 3 * (size_t)LPS_MAX_COLLIDERS || S * nbox > LPS_MAX_COLLIDERS ||
+        S * nsphere > LPS_MAX_COLLIDERS || S * nplane > LPS_MAX_COLLIDERS)
+        return LPS_ERR_INVALID;
+    for (size_t k = 0; k < S; k++) {
+        c->s_boxes[k].assign(boxes, boxes + nbox);
+        c->s_spheres[k].assign(spheres, spheres + nsphere);
+        c->s_planes[k].assign(planes, planes + nplane);
+    }
+    return upload_scenes(c);
 }
 
-int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* vhalf, const uint8_t* first,
-                      uint32_t n) {
-    if (!c || !x || n == 0 || n > LPS_MAX_PARTICLES) return LPS_ERR_INVALID;
+int lps_set_scene_colliders(lps_ctx* c, uint32_t scene, const lps_box* boxes, uint32_t nbox,
+                            const lps_sphere* spheres, uint32_t nsphere, const lps_plane* planes, uint32_t nplane) {
+    if (!c || scene >= c->h_sinfo.size()) return LPS_ERR_INVALID;
+    if ((nbox && !boxes) || (nsphere && !spheres) || (nplane && !planes)) return LPS_ERR_INVALID;
+    std::vector<lps_box> ob = c->s_boxes[scene];
+    std::vector<lps_sphere> os = c->s_spheres[scene];
+    std::vector<lps_plane> op = c->s_planes[scene];
+    c->s_boxes[scene].assign(boxes, boxes + nbox);
+    c->s_spheres[scene].assign(spheres, spheres + nsphere);
+    c->s_planes[scene].assign(planes, planes + nplane);
+    const int st = upload_scenes(c);
+    if (st == LPS_ERR_INVALID) {  // too many colliders in total: keep the previous set
+        c->s_boxes[scene] = ob;
+        c->s_spheres[scene] = os;
+        c->s_planes[scene] = op;
+    }
+    return st;
+}
+
+int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const float* gravity, const float* x,
+                   const float* v, const float* vhalf, const uint8_t* first) {
+    if (!c || !counts || !x || nscenes == 0 || nscenes > LPS_MAX_SCENES) return LPS_ERR_INVALID;
+    uint64_t total = 0, slots = 0;
+    for (uint32_t k = 0; k < nscenes; k++) {
+        if (counts[k] == 0) return LPS_ERR_INVALID;
+        total += counts[k];
+        slots += ((uint64_t)counts[k] + LPS_BLOCK - 1u) / LPS_BLOCK * LPS_BLOCK;
+    }
+    if (slots > LPS_MAX_PARTICLES) return LPS_ERR_INVALID;
+    if (gravity)
+        for (uint32_t k = 0; k < 3 * nscenes; k++)
+            if (!std::isfinite(gravity[k])) return LPS_ERR_INVALID;
+    const uint32_t n = (uint32_t)slots;
     int st = ensure_particles(c, n);
     if (st) return st;
-    if ((st = ensure_cells(c, cell_bound(c, x, n)))) return st;
+    // layout; colliders of scenes that existed before are kept
+    c->h_sinfo.assign(nscenes, SceneInfo{});
+    c->s_boxes.resize(nscenes);
+    c->s_spheres.resize(nscenes);
+    c->s_planes.resize(nscenes);
+    if (gravity) c->s_gravity.assign(gravity, gravity + 3 * nscenes);
+    else c->s_gravity.clear();
+    uint32_t slot = 0;
+    uint64_t cells = 0;
+    size_t off = 0;
+    for (uint32_t k = 0; k < nscenes; k++) {
+        SceneInfo& si = c->h_sinfo[k];
+        si.slot0 = slot;
+        si.count = counts[k];
+        si.padded = (counts[k] + LPS_BLOCK - 1u) / LPS_BLOCK * LPS_BLOCK;
+        si.block0 = slot / LPS_BLOCK;
+        slot += si.padded;
+        cells += scene_cell_bound(c, x + 3 * off, counts[k], c->s_boxes[k]) + 1u;
+        off += counts[k];
+    }
+    if ((st = ensure_cells(c, (uint32_t)std::min<uint64_t>(cells, 0x7FFFFFF0ull)))) return st;
+    if ((st = dalloc(&c->d_sinfo, nscenes))) return st;
+    if ((st = dalloc(&c->d_sgrid, nscenes))) return st;
+    if ((st = dalloc(&c->d_bscene, n / LPS_BLOCK))) return st;
+    if (c->h_sgrid) (void)hipHostFree(c->h_sgrid);
+    c->h_sgrid = nullptr;
+    HIPC(hipHostMalloc((void**)&c->h_sgrid, sizeof(SceneGrid) * nscenes, hipHostMallocDefault));
+    // particle arrays in slot order; empty slots at rest at the origin
     std::vector<float4> buf(n);
     auto up4 = [&](float4* dst, const float* src) -> int {
-        for (uint32_t i = 0; i < n; i++)
-            buf[i] = src ? make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+        size_t r0 = 0;  // index of the scene's first particle in the caller's arrays
+        for (uint32_t k = 0; k < nscenes; k++) {
+            const SceneInfo& si = c->h_sinfo[k];
+            for (uint32_t i = 0; i < si.padded; i++) {
+                const size_t r = r0 + i;
+                buf[si.slot0 + i] = (i < si.count && src)
+                                        ? make_float4(src[3 * r], src[3 * r + 1], src[3 * r + 2], 0.f)
+                                        : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+            r0 += si.count;
+        }
         HIPC(hipMemcpy(dst, buf.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
         return LPS_OK;
     };
@@ -679,37 +814,86 @@ int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* v
     if ((st = up4(c->vel[0], v))) return st;
     if ((st = up4(c->vh[0], vhalf))) return st;
     std::vector<uint8_t> f(n, 1);
-    if (first)
-        for (uint32_t i = 0; i < n; i++) f[i] = first[i] ? 1 : 0;
+    std::vector<uint32_t> ids(n, 0xFFFFFFFFu);
+    std::vector<uint8_t> bscene(n / LPS_BLOCK);
+    {
+        uint32_t r = 0;
+        for (uint32_t k = 0; k < nscenes; k++) {
+            const SceneInfo& si = c->h_sinfo[k];
+            for (uint32_t i = 0; i < si.count; i++, r++) {
+                ids[si.slot0 + i] = r;
+                if (first) f[si.slot0 + i] = first[r] ? 1 : 0;
+            }
+            for (uint32_t b = 0; b < si.padded / LPS_BLOCK; b++) bscene[si.block0 + b] = (uint8_t)k;
+        }
+    }
     HIPC(hipMemcpy(c->first[0], f.data(), n, hipMemcpyHostToDevice));
-    std::vector<uint32_t> ids(n);
-    for (uint32_t i = 0; i < n; i++) ids[i] = i;
     HIPC(hipMemcpy(c->id[0], ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->d_bscene, bscene.data(), n / LPS_BLOCK, hipMemcpyHostToDevice));
     HIPC(hipMemset(c->pprev[0], 0, sizeof(float) * n));
+    HIPC(hipMemset(c->d_sgrid, 0, sizeof(SceneGrid) * nscenes));
     c->n = n;
+    c->n_real = (uint32_t)total;
     c->have_step = false;
-    lps_dev::launch_bbox(c->stream, c->pos[0], n, c->bbox_part);
+    if ((st = upload_scenes(c))) return st;
+    lps_dev::launch_bbox(c->stream, c->pos[0], n, scenes(c), c->bbox_part);
     HIPC(hipStreamSynchronize(c->stream));
+    return LPS_OK;
+}
+
+int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* vhalf, const uint8_t* first,
+                      uint32_t n) {
+    if (!c || !x || n == 0 || n > LPS_MAX_PARTICLES - LPS_BLOCK) return LPS_ERR_INVALID;
+    return lps_set_scenes(c, 1u, &n, nullptr, x, v, vhalf, first);
+}
+
+int lps_get_scene_count(lps_ctx* c, uint32_t* nscenes) {
+    if (!c || !nscenes) return LPS_ERR_INVALID;
+    *nscenes = (uint32_t)c->h_sinfo.size();
+    return LPS_OK;
+}
+
+int lps_get_scene_stats(lps_ctx* c, lps_step_stats* stats, uint32_t nscenes) {
+    if (!c || !stats || nscenes != c->h_sinfo.size()) return LPS_ERR_INVALID;
+    if (!c->have_step) return LPS_ERR_STATE;
+    HIPC(hipMemcpyAsync(c->h_sgrid, c->d_sgrid, sizeof(SceneGrid) * nscenes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < nscenes; k++) {
+        const SceneGrid& sg = c->h_sgrid[k];
+        lps_step_stats& o = stats[k];
+        const uint32_t it = sg.stop_iter;
+        o.iterations = (int32_t)it;
+        o.max_density_error_ratio =
+            it ? __builtin_bit_cast(float, sg.err_bits[std::min(it - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u)]) /
+                     c->params.rest_density
+               : 0.f;
+        o.warm_start_used = c->seed_valid ? 1 : 0;
+        o.status = (int32_t)c->h_grid->status;
+        o.converged = (int32_t)sg.converged;
+    }
     return LPS_OK;
 }
 
 int lps_get_count(lps_ctx* c, uint32_t* n) {
     if (!c || !n) return LPS_ERR_INVALID;
-    *n = c->n;
+    *n = c->n_real;
     return LPS_OK;
 }
 
 int lps_get_particles(lps_ctx* c, float* x, float* v, float* vhalf, uint8_t* first, uint32_t* id, uint32_t n) {
-    if (!c || n != c->n || n == 0) return LPS_ERR_INVALID;
+    if (!c || n != c->n_real || n == 0) return LPS_ERR_INVALID;
     HIPC(hipStreamSynchronize(c->stream));
-    std::vector<float4> buf(n);
+    const uint32_t ns = c->n;
+    std::vector<float4> buf(ns), real(n);
     auto down4 = [&](float* dst, const float4* src) -> int {
         if (!dst) return LPS_OK;
-        HIPC(hipMemcpy(buf.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(buf.data(), src, sizeof(float4) * ns, hipMemcpyDeviceToHost));
+        compact_slots(c, buf, real.data(), 1);
         for (uint32_t i = 0; i < n; i++) {
-            dst[3 * i] = buf[i].x;
-            dst[3 * i + 1] = buf[i].y;
-            dst[3 * i + 2] = buf[i].z;
+            dst[3 * i] = real[i].x;
+            dst[3 * i + 1] = real[i].y;
+            dst[3 * i + 2] = real[i].z;
         }
         return LPS_OK;
     };
@@ -717,13 +901,22 @@ int lps_get_particles(lps_ctx* c, float* x, float* v, float* vhalf, uint8_t* fir
     if ((st = down4(x, c->pos[c->cur]))) return st;
     if ((st = down4(v, c->vel[c->cur]))) return st;
     if ((st = down4(vhalf, c->vh[c->cur]))) return st;
-    if (first) HIPC(hipMemcpy(first, c->first[c->cur], n, hipMemcpyDeviceToHost));
-    if (id) HIPC(hipMemcpy(id, c->id[c->cur], sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (first) {
+        std::vector<uint8_t> f(ns);
+        HIPC(hipMemcpy(f.data(), c->first[c->cur], ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, first, 1);
+    }
+    if (id) {
+        std::vector<uint32_t> f(ns);
+        HIPC(hipMemcpy(f.data(), c->id[c->cur], sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, id, 1);
+    }
     return LPS_OK;
 }
 
 int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
     if (!c) return LPS_ERR_INVALID;
+    if (c->n_real == 0) return LPS_ERR_STATE;
     if (c->n == 0) return LPS_ERR_STATE;
     if (!(dt > 0.f)) return LPS_ERR_INVALID;
     const bool pci = c->params.method == LPS_METHOD_PCISPH;
@@ -808,24 +1001,39 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
 int lps_get_grid(lps_ctx* c, lps_grid_info* info, uint32_t* keys, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* cell_end, uint32_t cap) {
     if (!c || !c->have_step) return LPS_ERR_STATE;
+    const uint32_t S = (uint32_t)c->h_sinfo.size();
     HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipMemcpyAsync(c->h_sgrid, c->d_sgrid, sizeof(SceneGrid) * S, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     const DevGrid& G = *c->h_grid;
+    const SceneGrid& S0 = c->h_sgrid[0];
+    // single scene: the grid of the reference; ensembles: scene 0's grid, the
+    // global (composite) keys and the global cell table
+    const uint32_t cells = S == 1 ? S0.cells : G.total_cells;
     if (info) {
         for (int a = 0; a < 3; a++) {
-            info->bbox_min[a] = G.bbox_min[a];
-            info->bbox_max[a] = G.bbox_max[a];
-            info->grid_offset[a] = G.offset[a];
-            info->grid_size[a] = G.gsize[a];
+            info->bbox_min[a] = S0.bbox_min[a];
+            info->bbox_max[a] = S0.bbox_max[a];
+            info->grid_offset[a] = S0.offset[a];
+            info->grid_size[a] = S0.gsize[a];
         }
         info->grid_spacing = G.spacing;
-        info->cell_count = G.cells;
+        info->cell_count = cells;
     }
-    const uint32_t n = c->n;
-    if (keys) HIPC(hipMemcpy(keys, c->skeys, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
-    if (perm) HIPC(hipMemcpy(perm, c->perm, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    const uint32_t ns = c->n;
+    if (keys || perm) {
+        std::vector<uint32_t> f(ns);
+        if (keys) {
+            HIPC(hipMemcpy(f.data(), c->skeys, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+            compact_slots(c, f, keys, 1);
+        }
+        if (perm) {
+            HIPC(hipMemcpy(f.data(), c->perm, sizeof(uint32_t) * ns, hipMemcpyDeviceToHost));
+            compact_slots(c, f, perm, 1);
+        }
+    }
     if (cell_start || cell_end) {
-        const uint32_t m = std::min(cap, G.cells);
+        const uint32_t m = std::min(cap, cells);
         std::vector<uint32_t> lb((size_t)m + 1);
         HIPC(hipMemcpy(lb.data(), c->cell_lb, sizeof(uint32_t) * ((size_t)m + 1), hipMemcpyDeviceToHost));
         for (uint32_t k = 0; k < m; k++) {
@@ -839,31 +1047,44 @@ int lps_get_grid(lps_ctx* c, lps_grid_info* info, uint32_t* keys, uint32_t* perm
 int lps_get_fields(lps_ctx* c, float* rho, float* pressure, float* force, float* rho_pred) {
     if (!c || !c->have_step) return LPS_ERR_STATE;
     HIPC(hipStreamSynchronize(c->stream));
-    const uint32_t n = c->n;
-    std::vector<float4> a(n), b;
-    if (rho || force) HIPC(hipMemcpy(a.data(), c->posrho, sizeof(float4) * n, hipMemcpyDeviceToHost));
+    const uint32_t ns = c->n, n = c->n_real;
+    std::vector<float4> a(ns), b, ra(n), rb;
+    std::vector<float> f(ns);
+    if (rho || force) {
+        HIPC(hipMemcpy(a.data(), c->posrho, sizeof(float4) * ns, hipMemcpyDeviceToHost));
+        compact_slots(c, a, ra.data(), 1);
+    }
     if (rho)
-        for (uint32_t i = 0; i < n; i++) rho[i] = a[i].w;
-    if (pressure) HIPC(hipMemcpy(pressure, c->press, sizeof(float) * n, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) rho[i] = ra[i].w;
+    if (pressure) {
+        HIPC(hipMemcpy(f.data(), c->press, sizeof(float) * ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, pressure, 1);
+    }
     if (force) {
-        HIPC(hipMemcpy(a.data(), c->force, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        HIPC(hipMemcpy(a.data(), c->force, sizeof(float4) * ns, hipMemcpyDeviceToHost));
+        compact_slots(c, a, ra.data(), 1);
         if (c->last_fp_valid) {
-            b.resize(n);
-            HIPC(hipMemcpy(b.data(), c->fp, sizeof(float4) * n, hipMemcpyDeviceToHost));
+            b.resize(ns);
+            rb.resize(n);
+            HIPC(hipMemcpy(b.data(), c->fp, sizeof(float4) * ns, hipMemcpyDeviceToHost));
+            compact_slots(c, b, rb.data(), 1);
         }
         for (uint32_t i = 0; i < n; i++) {
-            float fx = a[i].x, fy = a[i].y, fz = a[i].z;
+            float fx = ra[i].x, fy = ra[i].y, fz = ra[i].z;
             if (c->last_fp_valid) {
-                fx += b[i].x;
-                fy += b[i].y;
-                fz += b[i].z;
+                fx += rb[i].x;
+                fy += rb[i].y;
+                fz += rb[i].z;
             }
             force[3 * i] = fx;
             force[3 * i + 1] = fy;
             force[3 * i + 2] = fz;
         }
     }
-    if (rho_pred) HIPC(hipMemcpy(rho_pred, c->rho_pred, sizeof(float) * n, hipMemcpyDeviceToHost));
+    if (rho_pred) {
+        HIPC(hipMemcpy(f.data(), c->rho_pred, sizeof(float) * ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, rho_pred, 1);
+    }
     return LPS_OK;
 }
 
@@ -906,12 +1127,23 @@ int lps_mlp_capture(lps_ctx* c, int32_t on) {
 
 int lps_dump_pairs(lps_ctx* c, float* features, float* target, float* seed, uint32_t n) {
     if (!c || (!features && !target && !seed)) return LPS_ERR_INVALID;
-    if (n != c->n) return LPS_ERR_INVALID;
+    if (n != c->n_real) return LPS_ERR_INVALID;
     if (!c->have_step || !c->feat_valid || (seed && !c->seed_valid)) return LPS_ERR_STATE;
     HIPC(hipStreamSynchronize(c->stream));
-    if (features) HIPC(hipMemcpy(features, c->feat, sizeof(float) * LPS_MLP_IN * (size_t)n, hipMemcpyDeviceToHost));
-    if (target) HIPC(hipMemcpy(target, c->press, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
-    if (seed) HIPC(hipMemcpy(seed, c->seed, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    const uint32_t ns = c->n;
+    std::vector<float> f((size_t)ns * LPS_MLP_IN);
+    if (features) {
+        HIPC(hipMemcpy(f.data(), c->feat, sizeof(float) * LPS_MLP_IN * (size_t)ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, features, LPS_MLP_IN);
+    }
+    if (target) {
+        HIPC(hipMemcpy(f.data(), c->press, sizeof(float) * (size_t)ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, target, 1);
+    }
+    if (seed) {
+        HIPC(hipMemcpy(f.data(), c->seed, sizeof(float) * (size_t)ns, hipMemcpyDeviceToHost));
+        compact_slots(c, f, seed, 1);
+    }
     return LPS_OK;
 }
 

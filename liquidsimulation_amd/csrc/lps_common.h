@@ -16,31 +16,60 @@
 #define LPS_SCAN_ITEMS 16       // cells per thread in the cell-table scan
 #define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)
 #define LPS_MAX_ITER_SLOTS 64   // per-iteration error slots (iterations beyond reuse the last)
-#define LPS_MAX_COLLIDERS 64
+#define LPS_MAX_COLLIDERS 1024    // per context (all scenes)
 #define LPS_FEAT 16              // warm-start features per particle (LPS_MLP_IN)
 
-// Device-resident grid state of the current step.  Written by the bbox
-// reduction (integrate of the previous step, or the bbox kernel) and by the
-// grid set-up kernel; read by every later kernel of the step.
-struct DevGrid {
-    // grid of the current step (SPHSolver.cpp:1011-1040)
+// A context holds one or more independent scenes (an ensemble) that share
+// the fluid parameters.  Scene s owns the slot range [slot0, slot0 + padded)
+// of every particle array, padded to whole blocks of LPS_BLOCK slots, so each
+// particle block belongs to exactly one scene; its real particles always sit
+// in the first `count` slots (empty slots sort last, see below).
+#define LPS_MAX_SCENES 256
+
+// Per-scene constants, written by the host.
+struct SceneInfo {
+    uint32_t slot0;        // first slot (a multiple of LPS_BLOCK)
+    uint32_t count;        // real particles
+    uint32_t padded;       // slots reserved (count rounded up to LPS_BLOCK)
+    uint32_t block0;       // slot0 / LPS_BLOCK
+    float gravity[3];
+    uint32_t box0, nbox, sphere0, nsphere, plane0, nplane;  // ranges of the collider arrays
+};
+
+// Per-scene grid and PCISPH loop state of the current step, written on the
+// device (grid from the scene's bounding box, SPHSolver.cpp:1011-1040).  The
+// scene's cells are global cells [cell_base, cell_base + cells); global cell
+// cell_base + cells holds its empty slots.
+struct SceneGrid {
     float bbox_min[3];
     float bbox_max[3];
     float offset[3];
-    float spacing;
     uint32_t gsize[3];
     uint32_t cells;
-    uint32_t key_bits;       // bits of (cells - 1): radix passes beyond are skipped
-    uint32_t passes_done;    // radix passes executed this step
-    uint32_t status;         // lps_status of the step; kernels no-op when non-zero
-    uint32_t cells_needed;   // cell count requested when status == GRID_OVERFLOW
-    uint32_t err_bits[LPS_MAX_ITER_SLOTS];  // max |e_i| (float bits) per PCISPH correction
-    uint32_t stop_iter;      // correction after which the loop stops (0: running)
-    uint32_t steps_ok;       // steps whose grid set-up succeeded (monotone counter)
+    uint32_t cell_base;
+    uint32_t bad;            // 1: non-finite position, 2: extent too large for a grid
+    uint32_t stop_iter;      // correction after which this scene's loop stopped (0: running)
     uint32_t best_bits;      // best (smallest) max |e| of this step's corrections so far
     uint32_t stall;          // corrections since the best value last improved
     uint32_t converged;      // the tolerance was met
+    uint32_t err_bits[LPS_MAX_ITER_SLOTS];  // max |e_i| (float bits) per PCISPH correction
+};
+
+// Global state of the current step.
+struct DevGrid {
+    float spacing;           // cell size h (SPHSolver.cpp:1031)
+    uint32_t nscenes;
+    uint32_t total_cells;    // sum over scenes of cells + 1 (the empty-slot cell)
+    uint32_t key_bits;       // bits of (total_cells - 1): radix passes beyond are skipped
+    uint32_t passes_done;    // radix passes executed this step
+    uint32_t status;         // lps_status of the step; kernels no-op when non-zero
+    uint32_t cells_needed;   // total cells requested when status == GRID_OVERFLOW
+    uint32_t steps_ok;       // steps whose grid set-up succeeded, minus failed loops (monotone)
     uint32_t blocks_done;    // ticket counter of the last-block-done reduction
+    uint32_t running;        // scenes whose correction loop has not stopped
+    uint32_t iterations;     // corrections run this step (max over scenes)
+    uint32_t final_err;      // max over scenes of the last correction's max |e| (float bits)
+    uint32_t converged;      // every scene met the tolerance
 };
 
 // Per-block partial bounding box: orderable-uint min xyz, max xyz, non-finite flag.

@@ -12,6 +12,15 @@ struct LoopRule {
 
 namespace lps_dev {
 
+// The scenes of a context: constants, device grid/loop state, and the scene
+// of every particle block (blocks of LPS_BLOCK slots never straddle scenes).
+struct Scenes {
+    const SceneInfo* info;
+    SceneGrid* grid;
+    const uint8_t* block_scene;
+    uint32_t count;
+};
+
 struct ReorderArgs {
     const uint32_t *keysA, *valsA, *keysB, *valsB;
     uint32_t n;
@@ -31,6 +40,7 @@ struct ReorderArgs {
 
 // Buffers shared by the physics kernels of one step (sorted order).
 struct PhysArgs {
+    Scenes sc;
     float4* pos;          // x (w unused)
     float4* vel;          // v
     float4* vh;           // v_1/2
@@ -46,10 +56,11 @@ struct PhysArgs {
     SimConst c;
 };
 
-void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, uint32_t* part);
+void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, uint32_t* part);
 uint32_t bbox_parts(uint32_t n);
-void launch_grid_setup(hipStream_t s, DevGrid* g, const uint32_t* part, uint32_t n, float h, uint32_t cell_cap);
-void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, uint32_t* keys, uint32_t* vals);
+void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap);
+void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
+                 uint32_t* vals);
 uint32_t radix_tiles(uint32_t n);
 void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist, uint32_t* digit_total);
@@ -57,8 +68,7 @@ void launch_reorder(hipStream_t s, const ReorderArgs& a);
 uint32_t cell_blocks(uint32_t cell_cap);
 void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint32_t* block_max, uint32_t* cell_lb,
                        uint32_t cell_cap);
-void launch_density(hipStream_t s, const float4* pos, const DevGrid* g, const uint32_t* cell_lb, const SimConst& c,
-                    const float* tables, float4* posrho, float* press, float* q);
+void launch_density(hipStream_t s, const PhysArgs& a, float* press, float* q);
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat);
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
@@ -86,7 +96,8 @@ struct RowMap {
 // pressures are disabled, 0 when not finite), q_i = p_i / rho*_i^2,
 // seed_i = p_i.  No-op once the step has stopped or failed.
 void launch_mlp_seed(hipStream_t s, const MlpArgs& m, const float* feat, uint32_t n, float* press, float* q,
-                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g);
+                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
+                     const Scenes& sc);
 // batch inference (raw output, before tscale) / gradients, for training and tests
 void launch_mlp_forward(hipStream_t s, const MlpArgs& m, const float* x, uint32_t n, float* y);
 void launch_mlp_grad(hipStream_t s, const MlpArgs& m, const float* x, const float* t, const RowMap& rm, uint32_t n,

@@ -71,19 +71,46 @@ struct GridView {
     float off0, off1, off2, gs;
     int gx, gy, gz;
     uint32_t gxy;
+    uint32_t base;  // global index of the scene's cell 0
 };
 
-__device__ __forceinline__ GridView grid_view(const DevGrid* g) {
+__device__ __forceinline__ GridView grid_view(const DevGrid* g, const SceneGrid* sg) {
     GridView v;
-    v.off0 = g->offset[0];
-    v.off1 = g->offset[1];
-    v.off2 = g->offset[2];
+    v.off0 = sg->offset[0];
+    v.off1 = sg->offset[1];
+    v.off2 = sg->offset[2];
     v.gs = g->spacing;
-    v.gx = (int)g->gsize[0];
-    v.gy = (int)g->gsize[1];
-    v.gz = (int)g->gsize[2];
-    v.gxy = g->gsize[0] * g->gsize[1];
+    v.gx = (int)sg->gsize[0];
+    v.gy = (int)sg->gsize[1];
+    v.gz = (int)sg->gsize[2];
+    v.gxy = sg->gsize[0] * sg->gsize[1];
+    v.base = sg->cell_base;
     return v;
+}
+
+// The scene of a particle block and its constants; the block's real
+// particles are the slots below info.slot0 + info.count.
+struct BlockScene {
+    uint32_t s;
+    SceneInfo info;
+};
+__device__ __forceinline__ BlockScene block_scene(const lps_dev::Scenes& S, uint32_t lb) {
+    BlockScene b;
+    b.s = S.block_scene[lb];
+    b.info = S.info[b.s];
+    return b;
+}
+
+// The solver constants as seen by one scene: its gravity and collider counts.
+__device__ __forceinline__ SimConst scene_const(const SimConst& c, const SceneInfo& si) {
+    SimConst k = c;
+    k.gravity[0] = si.gravity[0];
+    k.gravity[1] = si.gravity[1];
+    k.gravity[2] = si.gravity[2];
+    k.nbox = si.nbox;
+    k.nsphere = si.nsphere;
+    k.nplane = si.nplane;
+    return k;
 }
 
 // The 27-cell stencil of the OpenCL kernels (e.g. SPHKernels.cl:219-244):
@@ -101,7 +128,7 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
         if (z < 0 || z >= G.gz) continue;
         for (int y = yg - 1; y <= yg + 1; y++) {
             if (y < 0 || y >= G.gy) continue;
-            const uint32_t base = (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+            const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
             const uint32_t b = lb[base + (uint32_t)x0];
             const uint32_t e = lb[base + (uint32_t)x1 + 1u];
             for (uint32_t j = b; j < e; j++) fn(j);
@@ -140,7 +167,7 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
         if (z < 0 || z >= G.gz) continue;
         for (int y = yg - 1; y <= yg + 1; y++) {
             if (y < 0 || y >= G.gy) continue;
-            const uint32_t base = (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+            const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
             const uint32_t b = lb[base + (uint32_t)x0];
             const uint32_t e = lb[base + (uint32_t)x1 + 1u];
             for (uint32_t w0 = b; w0 < e; w0 += 32u) {
@@ -388,22 +415,27 @@ __device__ __forceinline__ void bbox_partial(uint32_t* __restrict__ part, uint32
     }
 }
 
-__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, uint32_t n,
+__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, lps_dev::Scenes S,
                                                     uint32_t* __restrict__ part, uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    const BlockScene B = block_scene(S, lb);
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    const bool valid = i < B.info.slot0 + B.info.count;
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (i < n) p = pos[i];
-    bbox_partial(part, lb, i < n, p.x, p.y, p.z);
+    if (valid) p = pos[i];
+    bbox_partial(part, lb, valid, p.x, p.y, p.z);
 }
 
-// Grid parameters (SPHSolver.cpp:1031-1040).  min starts at FLT_MAX and max at
-// FLT_MIN as in the reference (SPHSolver.cpp:1013-1014).  Resets the per-step
-// counters.
-__global__ __launch_bounds__(LPS_BLOCK) void k_grid_setup(DevGrid* g, const uint32_t* __restrict__ part,
-                                                          uint32_t nparts, float h, uint32_t cell_cap) {
+// Grid parameters of each scene (SPHSolver.cpp:1031-1040), one block per
+// scene over the partial boxes of its particle blocks.  min starts at FLT_MAX
+// and max at FLT_MIN as in the reference (SPHSolver.cpp:1013-1014).
+__global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, const uint32_t* __restrict__ part,
+                                                          float h) {
     __shared__ uint32_t s_r[7][LPS_BLOCK / 64];
+    const uint32_t sidx = blockIdx.x;
+    const SceneInfo si = S.info[sidx];
+    const uint32_t nparts = si.padded / LPS_BLOCK;
     uint32_t r[7];
     for (int a = 0; a < 3; a++) {
         r[a] = f2o(FLT_MAX);
@@ -411,7 +443,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_setup(DevGrid* g, const uint
     }
     r[6] = 0u;
     for (uint32_t k = threadIdx.x; k < nparts; k += LPS_BLOCK) {
-        const uint32_t* q = part + k * LPS_BBOX_PART;
+        const uint32_t* q = part + (si.block0 + k) * LPS_BBOX_PART;
         for (int a = 0; a < 3; a++) {
             r[a] = min(r[a], q[a]);
             r[3 + a] = max(r[3 + a], q[3 + a]);
@@ -425,56 +457,92 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_setup(DevGrid* g, const uint
     if (threadIdx.x != 0) return;
     for (int a = 0; a < 7; a++)
         for (uint32_t k = 1; k < LPS_BLOCK / 64; k++) r[a] = (a < 3) ? min(r[a], s_r[a][k]) : max(r[a], s_r[a][k]);
-    uint32_t status = r[6] ? (uint32_t)LPS_ERR_NONFINITE : 0u;
+    SceneGrid* sg = S.grid + sidx;
+    uint32_t bad = r[6] ? 1u : 0u;
     float mn[3], mx[3];
+    const float gs = h * 1.f;
+    uint64_t cells = 1;
     for (int a = 0; a < 3; a++) {
         mn[a] = o2f(r[a]);
         mx[a] = o2f(r[3 + a]);
-        g->bbox_min[a] = mn[a];
-        g->bbox_max[a] = mx[a];
-    }
-    const float gs = h * 1.f;
-    g->spacing = gs;
-    uint64_t cells = 1;
-    for (int a = 0; a < 3; a++) {
-        g->offset[a] = -mn[a];
+        sg->bbox_min[a] = mn[a];
+        sg->bbox_max[a] = mx[a];
+        sg->offset[a] = -mn[a];
         const float ex = (mx[a] - mn[a]) / gs;
         if (!(ex < 2.0e9f)) {
-            status = status ? status : LPS_ERR_GRID_OVERFLOW;
-            g->gsize[a] = 1u;
+            bad = bad ? bad : 2u;
+            sg->gsize[a] = 1u;
         } else {
-            g->gsize[a] = (uint32_t)((int)ex + 1);
+            sg->gsize[a] = (uint32_t)((int)ex + 1);
         }
-        cells *= g->gsize[a];
+        cells *= sg->gsize[a];
     }
-    g->cells_needed = cells > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)cells;
-    if (!status && cells > (uint64_t)cell_cap) status = LPS_ERR_GRID_OVERFLOW;
-    g->cells = status ? 1u : (uint32_t)cells;
-    const uint32_t cm1 = g->cells - 1u;
+    if (cells >= 0x7FFFFFFFull) bad = bad ? bad : 2u;
+    sg->cells = bad ? 1u : (uint32_t)cells;
+    sg->bad = bad;
+}
+
+// Cell numbering across scenes (scene s owns cells [cell_base, cell_base +
+// cells] including its empty-slot cell), capacity check, and the per-step
+// reset of the loop state.
+__global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::Scenes S, float h,
+                                                           uint32_t cell_cap) {
+    const uint32_t t = threadIdx.x;
+    for (uint32_t sidx = t; sidx < S.count; sidx += LPS_BLOCK) {
+        SceneGrid* sg = S.grid + sidx;
+        sg->stop_iter = 0u;
+        sg->best_bits = 0u;
+        sg->stall = 0u;
+        sg->converged = 0u;
+        for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++) sg->err_bits[k] = 0u;
+    }
+    if (t != 0) return;
+    uint32_t status = 0u;
+    uint64_t total = 0;
+    for (uint32_t sidx = 0; sidx < S.count; sidx++) {
+        SceneGrid* sg = S.grid + sidx;
+        if (sg->bad == 1u) status = LPS_ERR_NONFINITE;
+        if (sg->bad == 2u && !status) status = LPS_ERR_GRID_OVERFLOW;
+        sg->cell_base = (uint32_t)min(total, (uint64_t)0xFFFFFFFFull);
+        total += (uint64_t)sg->cells + 1u;
+    }
+    g->spacing = h * 1.f;
+    g->cells_needed = total > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)total;
+    if (!status && total > (uint64_t)cell_cap) status = LPS_ERR_GRID_OVERFLOW;
+    g->total_cells = status ? 1u : (uint32_t)total;
+    const uint32_t cm1 = g->total_cells - 1u;
     g->key_bits = cm1 ? 32u - (uint32_t)__clz(cm1) : 0u;
     g->passes_done = 0u;
     g->status = status;
-    g->stop_iter = 0u;
-    g->best_bits = 0u;
-    g->stall = 0u;
-    g->converged = 0u;
     g->blocks_done = 0u;
+    g->running = S.count;
+    g->iterations = 0u;
+    g->final_err = 0u;
+    g->converged = 0u;
     if (!status) g->steps_ok += 1u;
-    for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++) g->err_bits[k] = 0u;
 }
 
-// calcGridIndices (SPHKernels.cl:66-82)
+// calcGridIndices (SPHKernels.cl:66-82) within the scene's grid, offset by the
+// scene's first cell; empty slots get the scene's empty-slot cell so they sort
+// after its particles.
 __global__ __launch_bounds__(LPS_BLOCK) void k_keys(const float4* __restrict__ pos, uint32_t n, const DevGrid* g,
-                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                                                    lps_dev::Scenes S, uint32_t* __restrict__ keys,
+                                                    uint32_t* __restrict__ vals) {
     if (step_dead(g)) return;
     const uint32_t i = blockIdx.x * LPS_BLOCK + threadIdx.x;
     if (i >= n) return;
-    const GridView G = grid_view(g);
-    const float4 p = pos[i];
-    const uint32_t xg = (uint32_t)truncf((p.x + G.off0) / G.gs);
-    const uint32_t yg = (uint32_t)truncf((p.y + G.off1) / G.gs);
-    const uint32_t zg = (uint32_t)truncf((p.z + G.off2) / G.gs);
-    keys[i] = xg + (uint32_t)G.gx * yg + G.gxy * zg;
+    const BlockScene B = block_scene(S, blockIdx.x);
+    const SceneGrid* sg = S.grid + B.s;
+    uint32_t key = sg->cell_base + sg->cells;
+    if (i < B.info.slot0 + B.info.count) {
+        const GridView G = grid_view(g, sg);
+        const float4 p = pos[i];
+        const uint32_t xg = (uint32_t)truncf((p.x + G.off0) / G.gs);
+        const uint32_t yg = (uint32_t)truncf((p.y + G.off1) / G.gs);
+        const uint32_t zg = (uint32_t)truncf((p.z + G.off2) / G.gs);
+        key = G.base + xg + (uint32_t)G.gx * yg + G.gxy * zg;
+    }
+    keys[i] = key;
     vals[i] = i;
 }
 
@@ -647,7 +715,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
 __global__ __launch_bounds__(LPS_BLOCK) void k_cell_reduce(const uint32_t* __restrict__ run_end, const DevGrid* g,
                                                            uint32_t* __restrict__ block_max) {
     if (step_dead(g)) return;
-    const uint32_t total = g->cells + 1u;
+    const uint32_t total = g->total_cells + 1u;
     const uint32_t base = blockIdx.x * LPS_SCAN_TILE;
     if (base >= total) return;
     uint32_t m = 0;
@@ -670,7 +738,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_cell_scan(uint32_t* __restrict__ 
                                                          const uint32_t* __restrict__ block_max,
                                                          uint32_t* __restrict__ cell_lb) {
     if (step_dead(g)) return;
-    const uint32_t total = g->cells + 1u;
+    const uint32_t total = g->total_cells + 1u;
     const uint32_t base = blockIdx.x * LPS_SCAN_TILE;
     if (base >= total) return;
     __shared__ uint32_t s[LPS_BLOCK / 64];
@@ -729,8 +797,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_cell_scan(uint32_t* __restrict__ 
 // force pass, the pressure, and q = p / rho^2 (the per-particle term of the
 // SPH pressure force, SPHKernels.cl:383,412).
 __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict__ pos, const DevGrid* g,
-                                                       const uint32_t* __restrict__ cell_lb, SimConst c,
-                                                       const float* __restrict__ tables,
+                                                       lps_dev::Scenes S, const uint32_t* __restrict__ cell_lb,
+                                                       SimConst c, const float* __restrict__ tables,
                                                        float4* __restrict__ posrho, float* __restrict__ press,
                                                        float* __restrict__ q, uint32_t nlogical) {
     if (step_dead(g)) return;
@@ -741,8 +809,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    if (i >= c.n) return;
-    const GridView G = grid_view(g);
+    const BlockScene B = block_scene(S, lb);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
     const float4 xi = pos[i];
     float sum = 0.f;
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
@@ -762,10 +831,10 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
 template <bool FEAT>
 __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
-    const uint8_t* __restrict__ first, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c,
-    const float* __restrict__ tables, const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres,
-    const lps_plane* __restrict__ planes, float4* __restrict__ force, float4* __restrict__ xpred,
-    const float* __restrict__ pprev, float* __restrict__ feat, uint32_t nlogical) {
+    const uint8_t* __restrict__ first, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
+    const float* __restrict__ tables, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
+    const lps_plane* __restrict__ planes0, float4* __restrict__ force, float4* __restrict__ xpred,
+    const float* __restrict__ pprev, float* __restrict__ feat, lps_dev::Scenes S, uint32_t nlogical) {
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
@@ -776,8 +845,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    if (i >= c.n) return;
-    const GridView G = grid_view(g);
+    const BlockScene B = block_scene(S, lb);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
+    const SimConst c = scene_const(c0, B.info);
+    const lps_box* boxes = boxes0 + B.info.box0;
+    const lps_sphere* spheres = spheres0 + B.info.sphere0;
+    const lps_plane* planes = planes0 + B.info.plane0;
     const float4 xi = posrho[i];
     const float4 vi = vel[i];
     const float ri = xi.w;
@@ -848,7 +922,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
                                                             const float* __restrict__ q, const DevGrid* g,
                                                             const uint32_t* __restrict__ cell_lb, SimConst c,
                                                             const float* __restrict__ tables,
-                                                            float4* __restrict__ force, uint32_t nlogical) {
+                                                            float4* __restrict__ force, lps_dev::Scenes S,
+                                                            uint32_t nlogical) {
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
@@ -857,8 +932,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    if (i >= c.n) return;
-    const GridView G = grid_view(g);
+    const BlockScene B = block_scene(S, lb);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
     const float4 xi = posrho[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
@@ -896,15 +972,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
     float* __restrict__ q, uint32_t k, LoopRule rule, float* __restrict__ feat, const float4* __restrict__ posrho,
-    uint32_t* __restrict__ host_flag, uint32_t nlogical) {
+    uint32_t* __restrict__ host_flag, lps_dev::Scenes S, uint32_t nlogical) {
     const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
-    if (step_dead(g) || g->stop_iter != 0u) {
+    if (step_dead(g) || g->running == 0u) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
-            const uint32_t st = step_dead(g) ? 2u : 1u;
-            const uint32_t si = g->stop_iter;
-            host_flag[0] = st;
-            host_flag[1] = si;
-            host_flag[2] = si ? g->err_bits[min(si - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u)] : 0u;
+            host_flag[0] = step_dead(g) ? 2u : 1u;
+            host_flag[1] = g->iterations;
+            host_flag[2] = g->final_err;
             host_flag[3] = g->converged;
         }
         return;
@@ -912,14 +986,24 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const uint32_t lb = xcd_block(nlogical);
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
-    __shared__ uint32_t s_last;
-    load_table(sW, tables);
-    load_thresholds(sT, tables);
+    __shared__ uint32_t s_last, s_run, s_err, s_conv, s_nan;
+    // blocks of scenes whose loop has stopped skip the correction but still
+    // take their ticket
+    bool active = false;
+    BlockScene B{};
+    if (lb != UINT_MAX) {
+        B = block_scene(S, lb);
+        active = S.grid[B.s].stop_iter == 0u;
+    }
+    if (active) {
+        load_table(sW, tables);
+        load_thresholds(sT, tables);
+    }
     __syncthreads();
-    const uint32_t i = (lb == UINT_MAX) ? UINT_MAX : lb * LPS_BLOCK + threadIdx.x;
+    const uint32_t i = active ? lb * LPS_BLOCK + threadIdx.x : UINT_MAX;
     uint32_t ebits = 0;
-    if (i < c.n) {
-        const GridView G = grid_view(g);
+    if (active && i < B.info.slot0 + B.info.count) {
+        const GridView G = grid_view(g, S.grid + B.s);
         const float4 xa = pos[i];
         const float4 xi = xpred[i];
         float sum = 0.f;
@@ -954,66 +1038,102 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     if (threadIdx.x == 0) {
         uint32_t m = 0;
         for (uint32_t w = 0; w < LPS_BLOCK / 64; w++) m = max(m, s_max[w]);
-        if (m) atomicMax(&g->err_bits[slot], m);
+        if (active && m) atomicMax(&S.grid[B.s].err_bits[slot], m);
         __threadfence();
         s_last = (atomicAdd(&g->blocks_done, 1u) == gridDim.x - 1u) ? 1u : 0u;
+        s_run = 0u;
+        s_err = 0u;
+        s_conv = 1u;
+        s_nan = 0u;
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
-    // last block: every block's maximum is visible (fenced before its ticket)
+    if (!s_last) return;
+    // last block: every block's maximum is visible (fenced before its ticket);
+    // one thread per running scene applies the loop rule
     __threadfence();
-    g->blocks_done = 0u;
-    const uint32_t eb = atomicOr(&g->err_bits[slot], 0u);
-    uint32_t state = 0;
-    if (eb >= 0x7F800000u) {
-        g->status = LPS_ERR_NONFINITE;
-        g->steps_ok -= 1u;  // counted by the grid setup; this step did not complete
-        state = 2;
-    } else {
+    for (uint32_t sidx = threadIdx.x; sidx < S.count; sidx += LPS_BLOCK) {
+        SceneGrid* sg = S.grid + sidx;
+        if (sg->stop_iter != 0u) {
+            atomicMax(&s_err, sg->err_bits[min(sg->stop_iter - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u)]);
+            if (!sg->converged) atomicAnd(&s_conv, 0u);
+            continue;
+        }
+        const uint32_t eb = atomicOr(&sg->err_bits[slot], 0u);
+        if (eb >= 0x7F800000u) {
+            atomicOr(&s_nan, 1u);
+            continue;
+        }
         // non-negative floats order like their bit patterns
-        if (k == 1u || eb < g->best_bits) {
-            g->best_bits = eb;
-            g->stall = 0u;
+        if (k == 1u || eb < sg->best_bits) {
+            sg->best_bits = eb;
+            sg->stall = 0u;
         } else {
-            g->stall += 1u;
+            sg->stall += 1u;
         }
         const float ratio = __uint_as_float(eb) / c.rest_density;
+        bool stop = false;
         if (k >= rule.min_iter && ratio < rule.tol) {
-            state = 1;
-            g->converged = 1u;
-        } else if (k >= rule.min_iter && rule.stall_limit > 0u && g->stall >= rule.stall_limit) {
-            state = 1;
+            stop = true;
+            sg->converged = 1u;
+        } else if (k >= rule.min_iter && rule.stall_limit > 0u && sg->stall >= rule.stall_limit) {
+            stop = true;
         } else if (k >= rule.max_iter) {
-            state = 1;
+            stop = true;
         }
-        if (state == 1u) g->stop_iter = k;
+        if (stop) sg->stop_iter = k;
+        else atomicAdd(&s_run, 1u);
+        atomicMax(&s_err, eb);
+        if (!sg->converged) atomicAnd(&s_conv, 0u);
     }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    g->blocks_done = 0u;
+    uint32_t state = 0u;
+    if (s_nan) {
+        g->status = LPS_ERR_NONFINITE;
+        g->steps_ok -= 1u;  // counted by the grid setup; this step did not complete
+        state = 2u;
+    } else if (s_run == 0u) {
+        state = 1u;
+    }
+    g->running = s_nan ? 0u : s_run;
+    g->iterations = k;
+    g->final_err = s_err;
+    g->converged = s_conv;
     host_flag[0] = state;
     host_flag[1] = k;
-    host_flag[2] = eb;
-    host_flag[3] = g->converged;
+    host_flag[2] = s_err;
+    host_flag[3] = s_conv;
 }
 
 // pciCalcPressureForce (SPHKernels.cl:806-878) followed by the next
 // prediction (pciIntegrate + pciHandleCollisions) from f + fp.
 __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, const float* __restrict__ rho_pred,
-    const float* __restrict__ q, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c,
+    const float* __restrict__ q, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
-    const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres,
-    const lps_plane* __restrict__ planes, float4* __restrict__ xpred_next, uint32_t k, uint32_t nlogical) {
-    // runs for correction k unless the loop stopped at an earlier correction
-    if (step_dead(g) || (g->stop_iter != 0u && g->stop_iter < k)) return;
+    const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
+    const lps_plane* __restrict__ planes0, float4* __restrict__ xpred_next, uint32_t k, lps_dev::Scenes S,
+    uint32_t nlogical) {
+    if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    // runs for correction k unless this scene's loop stopped at an earlier correction
+    const BlockScene B = block_scene(S, lb);
+    const uint32_t stop = S.grid[B.s].stop_iter;
+    if (stop != 0u && stop < k) return;
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
     load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    if (i >= c.n) return;
-    const GridView G = grid_view(g);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
+    const SimConst c = scene_const(c0, B.info);
+    const lps_box* boxes = boxes0 + B.info.box0;
+    const lps_sphere* spheres = spheres0 + B.info.sphere0;
+    const lps_plane* planes = planes0 + B.info.plane0;
     const float4 xa = pos[i];
     const float4 xi = xpred[i];
     const float tf = q[i];
@@ -1045,14 +1165,19 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
 __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     float4* __restrict__ pos, float4* __restrict__ vel, float4* __restrict__ vh, uint8_t* __restrict__ first,
     const float4* __restrict__ force, const float4* __restrict__ fp, const float4* __restrict__ posrho,
-    float* __restrict__ pprev, const float* __restrict__ press, DevGrid* g, SimConst c,
-    const lps_box* __restrict__ boxes, const lps_sphere* __restrict__ spheres, const lps_plane* __restrict__ planes,
-    uint32_t* __restrict__ bbox_part, uint32_t nlogical) {
+    float* __restrict__ pprev, const float* __restrict__ press, DevGrid* g, SimConst c0,
+    const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
+    const lps_plane* __restrict__ planes0, uint32_t* __restrict__ bbox_part, lps_dev::Scenes S, uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const bool dead = step_dead(g);
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    const bool valid = i < c.n;
+    const BlockScene B = block_scene(S, lb);
+    const bool valid = i < B.info.slot0 + B.info.count;
+    const SimConst c = scene_const(c0, B.info);
+    const lps_box* boxes = boxes0 + B.info.box0;
+    const lps_sphere* spheres = spheres0 + B.info.sphere0;
+    const lps_plane* planes = planes0 + B.info.plane0;
     float3 p = make_float3(0.f, 0.f, 0.f);
     if (valid) {
         const float4 x4 = pos[i];
@@ -1093,19 +1218,21 @@ namespace lps_dev {
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 static inline uint32_t xcd_grid(uint32_t nlogical) { return cdiv(nlogical, LPS_XCDS) * LPS_XCDS; }
 
-void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, uint32_t* part) {
+void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, uint32_t* part) {
     const uint32_t nl = cdiv(n, LPS_BLOCK);
-    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, n, part, nl);
+    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, sc, part, nl);
 }
 
 uint32_t bbox_parts(uint32_t n) { return cdiv(n, LPS_BLOCK); }
 
-void launch_grid_setup(hipStream_t s, DevGrid* g, const uint32_t* part, uint32_t n, float h, uint32_t cell_cap) {
-    k_grid_setup<<<1, LPS_BLOCK, 0, s>>>(g, part, cdiv(n, LPS_BLOCK), h, cell_cap);
+void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap) {
+    k_scene_grid<<<sc.count, LPS_BLOCK, 0, s>>>(sc, part, h);
+    k_grid_finish<<<1, LPS_BLOCK, 0, s>>>(g, sc, h, cell_cap);
 }
 
-void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, uint32_t* keys, uint32_t* vals) {
-    k_keys<<<cdiv(n, LPS_BLOCK), LPS_BLOCK, 0, s>>>(pos, n, g, keys, vals);
+void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
+                 uint32_t* vals) {
+    k_keys<<<cdiv(n, LPS_BLOCK), LPS_BLOCK, 0, s>>>(pos, n, g, sc, keys, vals);
 }
 
 uint32_t radix_tiles(uint32_t n) { return cdiv(n, LPS_RADIX_TILE); }
@@ -1135,22 +1262,21 @@ void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint3
     k_cell_scan<<<nb, LPS_BLOCK, 0, s>>>(run_end, g, block_max, cell_lb);
 }
 
-void launch_density(hipStream_t s, const float4* pos, const DevGrid* g, const uint32_t* cell_lb, const SimConst& c,
-                    const float* tables, float4* posrho, float* press, float* q) {
-    const uint32_t nl = cdiv(c.n, LPS_BLOCK);
-    k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, g, cell_lb, c, tables, posrho, press, q, nl);
+void launch_density(hipStream_t s, const PhysArgs& a, float* press, float* q) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.g, a.sc, a.cell_lb, a.c, a.tables, a.posrho, press, q, nl);
 }
 
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = feat ? k_forces<true> : k_forces<false>;
     kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables, a.boxes,
-                                            a.spheres, a.planes, a.force, xpred, pprev, feat, nl);
+                                            a.spheres, a.planes, a.force, xpred, pprev, feat, a.sc, nl);
 }
 
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    k_pressure_sph<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, nl);
+    k_pressure_sph<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, a.sc, nl);
 }
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
@@ -1158,7 +1284,7 @@ void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, f
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = (feat && k == 1u) ? k_pci_density<true> : k_pci_density<false>;
     kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, rule,
-                                            feat, a.posrho, host_flag, nl);
+                                            feat, a.posrho, host_flag, a.sc, nl);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
@@ -1166,14 +1292,14 @@ void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, 
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_pci_pressure<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp,
                                                       a.force, a.posrho, a.vel, a.vh, a.first, a.boxes, a.spheres,
-                                                      a.planes, xpred_next, k, nl);
+                                                      a.planes, xpred_next, k, a.sc, nl);
 }
 
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
                       uint32_t* bbox_part) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_integrate<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.vel, a.vh, a.first, a.force, fp, a.posrho, pprev, press,
-                                                   a.g, a.c, a.boxes, a.spheres, a.planes, bbox_part, nl);
+                                                   a.g, a.c, a.boxes, a.spheres, a.planes, bbox_part, a.sc, nl);
 }
 
 }  // namespace lps_dev

@@ -141,18 +141,24 @@ __device__ __forceinline__ void forward(const MlpSmem& S, const float* __restric
 __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(MlpArgs m, const float* __restrict__ feat, uint32_t n,
                                                         float* __restrict__ press, float* __restrict__ q,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
-                                                        float pscale, int clamp_nonneg, const DevGrid* g) {
-    if (g->status != 0u || g->stop_iter != 0u) return;
+                                                        float pscale, int clamp_nonneg, const DevGrid* g,
+                                                        lps_dev::Scenes sc) {
+    if (g->status != 0u || g->running == 0u) return;
     __shared__ MlpSmem S;
     load_weights(S, m, false);
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    for (uint32_t bb = blockIdx.x * LPS_BLOCK; bb < n; bb += gridDim.x * LPS_BLOCK)
+    // one 256-slot chunk is one particle block, so it belongs to one scene;
+    // chunks of scenes that stopped at the first correction are skipped
+    for (uint32_t bb = blockIdx.x * LPS_BLOCK; bb < n; bb += gridDim.x * LPS_BLOCK) {
+        const uint32_t s = sc.block_scene[bb / LPS_BLOCK];
+        if (sc.grid[s].stop_iter != 0u) continue;
+        const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
             Fwd F;
-            forward(S, feat + (size_t)min(p, n - 1u) * IN, p < n, F);
-            if (p < n && (l >> 4) == 0u) {
+            forward(S, feat + (size_t)min(p, n - 1u) * IN, p < end, F);
+            if (p < end && (l >> 4) == 0u) {
                 float v = F.y * m.tscale * pscale;
                 if (!isfinite(v)) v = 0.f;
                 if (clamp_nonneg) v = fmaxf(v, 0.f);
@@ -162,6 +168,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(MlpArgs m, const float* 
                 seed[p] = v;
             }
         }
+    }
 }
 
 __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(MlpArgs m, const float* __restrict__ x, uint32_t n,
@@ -372,8 +379,9 @@ uint32_t mlp_grid(uint64_t n) {
 }  // namespace
 
 void launch_mlp_seed(hipStream_t s, const MlpArgs& m, const float* feat, uint32_t n, float* press, float* q,
-                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g) {
-    k_mlp_seed<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(m, feat, n, press, q, seed, rho_pred, pscale, clamp_nonneg, g);
+                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
+                     const Scenes& sc) {
+    k_mlp_seed<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(m, feat, n, press, q, seed, rho_pred, pscale, clamp_nonneg, g, sc);
 }
 
 void launch_mlp_forward(hipStream_t s, const MlpArgs& m, const float* x, uint32_t n, float* y) {

@@ -360,5 +360,52 @@ class PCISPHSolver(SPHSolver):
         return self._params.max_density_error_ratio
 
 
+class Ensemble(PCISPHSolver):
+    """Many independent scenes in one context (no counterpart in the
+    reference): shared fluid parameters, per-scene gravity and colliders, one
+    launch per phase for all scenes.  Each scene's results equal those of the
+    scene run alone.  Works for SPH and PCISPH (params.method)."""
+
+    def set_scenes(self, xs, vs=None, gravity=None):
+        """xs: list of [n_s, 3] position arrays; vs: matching velocities or None;
+        gravity: [S, 3] or None (the parameters' gravity for every scene)."""
+        S = len(xs)
+        counts = np.array([np.asarray(x).reshape(-1, 3).shape[0] for x in xs], np.uint32)
+        x = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1, 3) for a in xs]))
+        v = None
+        if vs is not None:
+            v = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32).reshape(-1, 3) for a in vs]))
+        g = None if gravity is None else np.ascontiguousarray(np.asarray(gravity, np.float32).reshape(S, 3))
+        self._push_colliders()
+        check(lib().lps_set_scenes(self._ctx, S, counts, ptr(g), x.reshape(-1), ptr(v), None, None),
+              "lps_set_scenes")
+        self._n = int(counts.sum())
+        self._counts = counts
+        self._pending = []
+
+    @property
+    def scene_count(self) -> int:
+        n = C.c_uint32()
+        check(lib().lps_get_scene_count(self._ctx, C.byref(n)), "lps_get_scene_count")
+        return n.value
+
+    def scene_slices(self):
+        """Slices of each scene's particles in the concatenated arrays."""
+        ends = np.cumsum(self._counts)
+        return [slice(int(e - c), int(e)) for c, e in zip(self._counts, ends)]
+
+    def set_scene_colliders(self, scene: int, col: Colliders):
+        boxes, nb, spheres, ns, planes, npl = col.c_arrays()
+        check(lib().lps_set_scene_colliders(self._ctx, int(scene), C.cast(boxes, C.c_void_p), nb,
+                                            C.cast(spheres, C.c_void_p), ns, C.cast(planes, C.c_void_p), npl),
+              "lps_set_scene_colliders")
+
+    def scene_stats(self):
+        S = self.scene_count
+        st = (CStepStats * S)()
+        check(lib().lps_get_scene_stats(self._ctx, C.cast(st, C.c_void_p), S), "lps_get_scene_stats")
+        return list(st)
+
+
 def solver_for(params: Params):
     return PCISPHSolver(params) if params.method == METHOD_PCISPH else SPHSolver(params)
