@@ -125,6 +125,99 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None):
             "energy": energy, "rerun_identical": same}
 
 
+def run_ensemble(ens, steps: int, warmup: int, weights=None, profile: bool = False):
+    """All scenes of an ensemble in one context: warm-up, then `steps` timed
+    steps (HIP events on the library stream)."""
+    import torch
+    from liquidsimulation_amd.solver import Ensemble
+    from liquidsimulation_amd.abi import WARM_LEARNED, WARM_NONE
+    from liquidsimulation_amd import warm_start
+    e = Ensemble(ens.params.replace(warm_start=WARM_LEARNED if weights is not None else WARM_NONE))
+    if weights is not None:
+        warm_start.apply(e, weights)
+    e.set_scenes([m.x for m in ens.members], [m.v for m in ens.members], gravity=ens.gravity)
+    for k, m in enumerate(ens.members):
+        e.set_scene_colliders(k, m.colliders)
+    if warmup:
+        e.update(ens.dt, warmup)
+    stream = torch.cuda.ExternalStream(e.stream())
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    stats = e.update(ens.dt, steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    last = e.scene_stats()
+    kernels = {}
+    if profile:
+        e.profile(True)
+        e.profile_reset()
+        e.update(ens.dt, min(steps, 50))
+        for name in KERNELS:
+            n, tot = e.profile_query(name)
+            if n:
+                kernels[name] = 1000.0 * tot / n
+        e.profile(False)
+    e.close()
+    its = np.array([x.iterations for x in stats], np.int64)
+    return {"value": ens.n * steps / (ms / 1000.0), "ms_per_step": ms / steps,
+            "iterations_max_over_scenes": {"mean": float(its.mean()), "max": int(its.max())},
+            "seeded_steps": int(sum(x.warm_start_used for x in stats)),
+            "last_step_scene_iterations": [x.iterations for x in last],
+            "nonconverged_scenes_last_step": int(sum(1 - x.converged for x in last)),
+            "kernels_avg_us": kernels}
+
+
+def run_members_alone(ens, steps: int, warmup: int):
+    """The same scenes one context at a time (rate over `steps` steps each)."""
+    import torch
+    total_ms = 0.0
+    for k, m in enumerate(ens.members):
+        sc = m.with_params(gravity=ens.gravity[k])
+        s = _solver(sc, None)
+        if warmup:
+            s.update(sc.dt, warmup)
+        stream = torch.cuda.ExternalStream(s.stream())
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        s.update(sc.dt, steps)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        total_ms += ev0.elapsed_time(ev1)
+        s.close()
+    return {"value": ens.n * steps / (total_ms / 1000.0), "ms_per_step": total_ms / steps,
+            "steps_per_scene": steps}
+
+
+def bench_ensemble(args):
+    from liquidsimulation_amd import scenes, warm_start
+    ens = scenes.WORKLOADS[args.workload]()
+    steps = args.steps if args.steps_given else ens.steps
+    w = warm_start.load() if (args.mode == "both" and os.path.exists(warm_start.WEIGHTS)) else None
+    plain = run_ensemble(ens, steps, args.warmup, None, not args.no_profile)
+    learned = run_ensemble(ens, steps, args.warmup, w, False) if w is not None else None
+    alone = run_members_alone(ens, max(20, steps // 10), args.warmup)
+    head = learned if learned is not None else plain
+    out = {
+        "metric": "particle-steps/s of an ensemble of independent PCISPH dam breaks in one context, 1 GPU",
+        "value": head["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (seeded dam-break lattices, per-scene block dims, gravity, sphere)",
+        "config": {"workload": ens.name, "scenes": len(ens.members), "particles": ens.n,
+                   "particles_min": int(min(m.n for m in ens.members)),
+                   "particles_max": int(max(m.n for m in ens.members)), "dt": ens.dt,
+                   "warm_start": "learned" if learned is not None else "none"},
+        "plain": plain, "learned": learned,
+        "scenes_one_at_a_time": alone,
+        "speedup_batched_vs_one_at_a_time": plain["value"] / alone["value"],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def mode_summary(res, scene, steps):
     its, errs = res["its"], res["errs"]
     nonconv = int(np.sum(res["conv"] == 0)) if scene.params.method == 1 else 0
@@ -213,6 +306,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy five times (scales the HBM byte counters)")
     args = ap.parse_args()
+    args.steps_given = "--steps" in sys.argv
     if args.copy_calibration:
         copy_calibration()
         return
@@ -223,6 +317,9 @@ def main():
 
     from liquidsimulation_amd import scenes, warm_start
     from liquidsimulation_amd.abi import WARM_LEARNED, WARM_NONE
+    if args.workload.startswith("ensemble"):
+        bench_ensemble(args)
+        return
     scene = scenes.WORKLOADS[args.workload]()
     learned = None
     if args.mode == "both" and scene.params.method == 1 and os.path.exists(warm_start.WEIGHTS):

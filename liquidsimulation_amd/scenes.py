@@ -212,9 +212,54 @@ def small_block(nx=8, ny=8, nz=8, s=0.02, jitter_amp=1e-4, seed=5, method=METHOD
                      steps=10, name=f"small_{nx}x{ny}x{nz}", **kw)
 
 
+@dataclass
+class EnsembleScene:
+    """Independent scenes stepped together (shared fluid parameters)."""
+    name: str
+    params: Params
+    members: List[Scene]            # per-scene positions, velocities, colliders
+    gravity: List[tuple]            # per-scene gravity
+    dt: float
+    steps: int
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(sum(m.n for m in self.members))
+
+
+def ensemble_64(seed: int = 4, count: int = 64, n_min: int = 8192, n_max: int = 131072) -> EnsembleScene:
+    """Batched ensemble: `count` dam breaks with particle counts log-uniform in
+    [n_min, n_max], drawn from splitmix64(seed): per-scene block dims (ny = 2 nx,
+    nz in [nx/2, 2 nx]), gravity tilted by up to +-10 degrees about z and x, one
+    obstacle sphere in the collapse path; spacing 0.00625 m and the 524k scene's
+    fluid and solver parameters, dt 0.001 s, 500 steps."""
+    s_ = 0.00625
+    u = uniform01(seed, 8 * count).astype(np.float64).reshape(count, 8)
+    members, gravity = [], []
+    for k in range(count):
+        n_t = math.exp(math.log(n_min) + (math.log(n_max) - math.log(n_min)) * u[k, 0])
+        rz = 0.5 + 1.5 * u[k, 1]                       # nz / nx
+        nx = max(4, int(round((n_t / (2.0 * rz)) ** (1.0 / 3.0))))
+        ny, nz = 2 * nx, max(4, int(round(rz * nx)))
+        bx, by, bz = 4.0 * s_ * (nx + 1), 1.5 * s_ * (ny + 1), s_ * (nz + 3)
+        sc = dam_break(nx, ny, nz, s_, box=(bx, by, bz), jitter_amp=1e-5, seed=seed * 1000 + k,
+                       method=METHOD_PCISPH, dt=0.001, steps=500, name=f"member_{k}")
+        r = (0.15 + 0.15 * u[k, 2]) * min(by, bz)
+        sc.colliders.spheres = [Sphere(center=(bx * (0.55 + 0.25 * u[k, 3]), r * (0.6 + 0.8 * u[k, 4]), bz * 0.5),
+                                       radius=r, boundary=False)]
+        tz, tx = math.radians(20.0 * (u[k, 5] - 0.5)), math.radians(20.0 * (u[k, 6] - 0.5))
+        g = (9.81 * math.sin(tz), -9.81 * math.cos(tz) * math.cos(tx), 9.81 * math.cos(tz) * math.sin(tx))
+        members.append(sc)
+        gravity.append(tuple(float(v) for v in g))
+    return EnsembleScene(name="ensemble_64", params=members[0].params, members=members, gravity=gravity,
+                         dt=0.001, steps=500, meta={"seed": seed, "count": count})
+
+
 WORKLOADS = {
     "sph_dam_break_16k": sph_dam_break_16k,
     "pcisph_dam_break_16k": pcisph_dam_break_16k,
     "pcisph_dam_break_524k": pcisph_dam_break_524k,
     "pcisph_obstacles_2m": pcisph_obstacles_2m,
+    "ensemble_64": ensemble_64,
 }
