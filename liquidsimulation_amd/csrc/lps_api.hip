@@ -77,6 +77,7 @@ struct lps_ctx {
     SceneGrid* d_sgrid = nullptr;
     SceneGrid* h_sgrid = nullptr;          // pinned copy for per-scene statistics
     uint8_t* d_bscene = nullptr;           // scene of every particle block
+    uint32_t* d_actlist = nullptr;         // [2][blocks] active-block lists of the correction kernels
 
     // host
     uint32_t* flags = nullptr;  // pinned, coherent: 4 words per iteration slot
@@ -322,7 +323,7 @@ void free_all(lps_ctx* c) {
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
     dfree(c->d_mlp); dfree(c->feat); dfree(c->seed); dfree(c->coll_x); dfree(c->coll_t);
     dfree(c->d_grad); dfree(c->d_m1); dfree(c->d_m2); dfree(c->d_loss); dfree(c->d_stats);
-    dfree(c->d_sinfo); dfree(c->d_sgrid); dfree(c->d_bscene);
+    dfree(c->d_sinfo); dfree(c->d_sgrid); dfree(c->d_bscene); dfree(c->d_actlist);
     if (c->h_sgrid) (void)hipHostFree(c->h_sgrid);
     c->h_sgrid = nullptr;
     if (c->flags) (void)hipHostFree(c->flags);
@@ -332,7 +333,8 @@ void free_all(lps_ctx* c) {
 }
 
 lps_dev::Scenes scenes(const lps_ctx* c) {
-    return lps_dev::Scenes{c->d_sinfo, c->d_sgrid, c->d_bscene, (uint32_t)c->h_sinfo.size()};
+    return lps_dev::Scenes{c->d_sinfo, c->d_sgrid, c->d_bscene, (uint32_t)c->h_sinfo.size(), c->n / LPS_BLOCK,
+                           c->d_actlist};
 }
 
 lps_dev::PhysArgs phys_args(lps_ctx* c, const SimConst& k) {
@@ -789,6 +791,7 @@ int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const f
     if ((st = dalloc(&c->d_sinfo, nscenes))) return st;
     if ((st = dalloc(&c->d_sgrid, nscenes))) return st;
     if ((st = dalloc(&c->d_bscene, n / LPS_BLOCK))) return st;
+    if ((st = dalloc(&c->d_actlist, 2 * (size_t)(n / LPS_BLOCK)))) return st;
     if (c->h_sgrid) (void)hipHostFree(c->h_sgrid);
     c->h_sgrid = nullptr;
     HIPC(hipHostMalloc((void**)&c->h_sgrid, sizeof(SceneGrid) * nscenes, hipHostMallocDefault));

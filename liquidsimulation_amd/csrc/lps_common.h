@@ -70,6 +70,13 @@ struct DevGrid {
     uint32_t iterations;     // corrections run this step (max over scenes)
     uint32_t final_err;      // max over scenes of the last correction's max |e| (float bits)
     uint32_t converged;      // every scene met the tolerance
+    // Correction kernels run only the particle blocks of scenes whose loop
+    // is still running.  Two block lists alternate; list_of[k % 64] names the
+    // list of correction k (both of its passes).  Entries are written once per
+    // correction, ahead of it, so a workgroup that starts late never sees a
+    // list change under it.
+    uint32_t act_count[2];   // blocks in list 0 / 1
+    uint32_t list_of[LPS_MAX_ITER_SLOTS];
 };
 
 // Per-block partial bounding box: orderable-uint min xyz, max xyz, non-finite flag.

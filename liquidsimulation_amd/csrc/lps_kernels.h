@@ -19,6 +19,8 @@ struct Scenes {
     SceneGrid* grid;
     const uint8_t* block_scene;
     uint32_t count;
+    uint32_t nblocks;        // particle blocks (slots / LPS_BLOCK)
+    uint32_t* act_list;      // [2][nblocks] active-block lists of the correction kernels
 };
 
 struct ReorderArgs {

@@ -101,6 +101,18 @@ __device__ __forceinline__ BlockScene block_scene(const lps_dev::Scenes& S, uint
     return b;
 }
 
+// Particle block of this workgroup among the active blocks of list L, with
+// the active range split XCD-contiguously over the 8 XCDs; UINT_MAX when the
+// workgroup has no block.
+__device__ __forceinline__ uint32_t active_block(const DevGrid* g, const lps_dev::Scenes& S, uint32_t L) {
+    const uint32_t act = g->act_count[L];
+    const uint32_t per = (act + LPS_XCDS - 1) / LPS_XCDS;
+    const uint32_t b = blockIdx.x;
+    if (b / LPS_XCDS >= per) return UINT_MAX;
+    const uint32_t lb = (b % LPS_XCDS) * per + b / LPS_XCDS;
+    return lb < act ? S.act_list[L * S.nblocks + lb] : UINT_MAX;
+}
+
 // The solver constants as seen by one scene: its gravity and collider counts.
 __device__ __forceinline__ SimConst scene_const(const SimConst& c, const SceneInfo& si) {
     SimConst k = c;
@@ -488,6 +500,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, con
 __global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::Scenes S, float h,
                                                            uint32_t cell_cap) {
     const uint32_t t = threadIdx.x;
+    for (uint32_t b = t; b < S.nblocks; b += LPS_BLOCK) S.act_list[b] = b;
     for (uint32_t sidx = t; sidx < S.count; sidx += LPS_BLOCK) {
         SceneGrid* sg = S.grid + sidx;
         sg->stop_iter = 0u;
@@ -519,6 +532,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::
     g->iterations = 0u;
     g->final_err = 0u;
     g->converged = 0u;
+    g->act_count[0] = S.nblocks;
+    g->act_count[1] = 0u;
+    g->list_of[1] = 0u;
     if (!status) g->steps_ok += 1u;
 }
 
@@ -983,22 +999,18 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         }
         return;
     }
-    const uint32_t lb = xcd_block(nlogical);
+    // only blocks of scenes whose loop is running are scheduled
+    const uint32_t L = g->list_of[k % LPS_MAX_ITER_SLOTS];
+    const uint32_t nact = g->act_count[L];
+    const uint32_t lb = active_block(g, S, L);
+    if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
-    __shared__ uint32_t s_last, s_run, s_err, s_conv, s_nan;
-    // blocks of scenes whose loop has stopped skip the correction but still
-    // take their ticket
-    bool active = false;
-    BlockScene B{};
-    if (lb != UINT_MAX) {
-        B = block_scene(S, lb);
-        active = S.grid[B.s].stop_iter == 0u;
-    }
-    if (active) {
-        load_table(sW, tables);
-        load_thresholds(sT, tables);
-    }
+    __shared__ uint32_t s_last, s_run, s_err, s_conv, s_nan, s_nb;
+    const BlockScene B = block_scene(S, lb);
+    const bool active = true;
+    load_table(sW, tables);
+    load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = active ? lb * LPS_BLOCK + threadIdx.x : UINT_MAX;
     uint32_t ebits = 0;
@@ -1040,7 +1052,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         for (uint32_t w = 0; w < LPS_BLOCK / 64; w++) m = max(m, s_max[w]);
         if (active && m) atomicMax(&S.grid[B.s].err_bits[slot], m);
         __threadfence();
-        s_last = (atomicAdd(&g->blocks_done, 1u) == gridDim.x - 1u) ? 1u : 0u;
+        s_last = (atomicAdd(&g->blocks_done, 1u) == nact - 1u) ? 1u : 0u;
         s_run = 0u;
         s_err = 0u;
         s_conv = 1u;
@@ -1086,7 +1098,27 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         if (!sg->converged) atomicAnd(&s_conv, 0u);
     }
     __syncthreads();
+    // the pressure pass of this correction uses this list; the next
+    // correction uses the blocks of the scenes still running
+    uint32_t next = L;
+    if (s_run != g->running) {
+        const uint32_t N = 1u - L;
+        if (threadIdx.x == 0) s_nb = 0u;
+        __syncthreads();
+        for (uint32_t sidx = 0; sidx < S.count; sidx++) {
+            if (S.grid[sidx].stop_iter != 0u) continue;
+            const SceneInfo& si = S.info[sidx];
+            const uint32_t nb = si.padded / LPS_BLOCK, at = s_nb;
+            for (uint32_t b = threadIdx.x; b < nb; b += LPS_BLOCK) S.act_list[N * S.nblocks + at + b] = si.block0 + b;
+            __syncthreads();
+            if (threadIdx.x == 0) s_nb = at + nb;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) g->act_count[N] = s_nb;
+        next = N;
+    }
     if (threadIdx.x != 0) return;
+    g->list_of[(k + 1u) % LPS_MAX_ITER_SLOTS] = next;
     g->blocks_done = 0u;
     uint32_t state = 0u;
     if (s_nan) {
@@ -1117,9 +1149,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const lps_plane* __restrict__ planes0, float4* __restrict__ xpred_next, uint32_t k, lps_dev::Scenes S,
     uint32_t nlogical) {
     if (step_dead(g)) return;
-    const uint32_t lb = xcd_block(nlogical);
+    // the blocks of the scenes that ran this correction's predicted-density pass
+    const uint32_t lb = active_block(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS]);
     if (lb == UINT_MAX) return;
-    // runs for correction k unless this scene's loop stopped at an earlier correction
     const BlockScene B = block_scene(S, lb);
     const uint32_t stop = S.grid[B.s].stop_iter;
     if (stop != 0u && stop < k) return;
