@@ -82,6 +82,7 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None):
     s = _solver(scene, weights)
     if warmup:
         s.update(scene.dt, warmup)
+    tuning = s.autotune(scene.dt, 10) if scene.params.method == 1 else None
     stream = torch.cuda.ExternalStream(s.stream())
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -111,6 +112,8 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None):
         s = _solver(scene, weights)
         if warmup:
             s.update(scene.dt, warmup)
+        if tuning:
+            s.set_tuning(tuning["use_graphs"], tuning["lookahead"])
         s.profile(True)
         s.profile_reset()
         rerun = s.update(scene.dt, steps)
@@ -122,7 +125,7 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None):
         same = bool(np.array_equal(np.array([x.iterations for x in rerun], np.int64), its))
         s.close()
     return {"ms": ms, "wall": wall, "its": its, "errs": errs, "conv": conv, "used": used, "kernels": kernels,
-            "energy": energy, "rerun_identical": same}
+            "energy": energy, "rerun_identical": same, "tuning": tuning}
 
 
 def run_ensemble(ens, steps: int, warmup: int, weights=None, profile: bool = False):
@@ -140,6 +143,7 @@ def run_ensemble(ens, steps: int, warmup: int, weights=None, profile: bool = Fal
         e.set_scene_colliders(k, m.colliders)
     if warmup:
         e.update(ens.dt, warmup)
+    tuning = e.autotune(ens.dt, 5)
     stream = torch.cuda.ExternalStream(e.stream())
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -167,7 +171,7 @@ def run_ensemble(ens, steps: int, warmup: int, weights=None, profile: bool = Fal
             "seeded_steps": int(sum(x.warm_start_used for x in stats)),
             "last_step_scene_iterations": [x.iterations for x in last],
             "nonconverged_scenes_last_step": int(sum(1 - x.converged for x in last)),
-            "kernels_avg_us": kernels}
+            "kernels_avg_us": kernels, "tuning": tuning}
 
 
 def run_members_alone(ens, steps: int, warmup: int):
@@ -236,6 +240,7 @@ def mode_summary(res, scene, steps):
         "end_energy": res["energy"],
         "kernels_avg_us": {k: v["avg_us"] for k, v in res["kernels"].items()},
         "profiled_rerun_identical": res["rerun_identical"],
+        "tuning": res["tuning"],
     }
 
 

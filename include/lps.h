@@ -155,6 +155,21 @@ int lps_get_scene_count(lps_ctx* ctx, uint32_t* nscenes);
 /* Per-scene statistics of the last step (iterations, final error ratio, converged). */
 int lps_get_scene_stats(lps_ctx* ctx, lps_step_stats* stats /* [nscenes] */, uint32_t nscenes);
 
+/* Launch tuning (no counterpart in the reference).  use_graphs: replay the
+ * fixed part of a PCISPH step (grid build, density, forces, corrections
+ * 1..min) as a captured HIP graph; lookahead: corrections kept queued beyond
+ * the one whose convergence result the host waits for.  lps_autotune times
+ * `steps` steps of every candidate from the current state, restores the state
+ * (the run continues exactly as without tuning) and keeps the fastest. */
+typedef struct lps_tuning {
+    int32_t use_graphs;   /* 0 or 1 */
+    uint32_t lookahead;   /* 1..4 */
+    float ms_per_step;    /* lps_autotune: measured time of the chosen setting */
+} lps_tuning;
+int lps_set_tuning(lps_ctx* ctx, const lps_tuning* tuning);
+int lps_get_tuning(lps_ctx* ctx, lps_tuning* tuning);
+int lps_autotune(lps_ctx* ctx, float dt, uint32_t steps, lps_tuning* chosen);
+
 /* One or more steps of PhysicSolver::update (PhysicSolver.cpp:12-21) on the device.
  * stats may be NULL or hold nsteps entries.  Returns the first failing status. */
 int lps_step(lps_ctx* ctx, float dt, uint32_t nsteps, lps_step_stats* stats);

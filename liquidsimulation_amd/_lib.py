@@ -8,7 +8,7 @@ import re
 
 import numpy as np
 
-from .abi import CGridInfo, CParams, CStepStats, CBox, CSphere, CPlane, CTrainCfg
+from .abi import CGridInfo, CParams, CStepStats, CBox, CSphere, CPlane, CTrainCfg, CTuning
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liblps.so")
@@ -57,6 +57,9 @@ def lib():
     L.lps_get_scene_count.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.lps_get_scene_stats.argtypes = [vp, vp, C.c_uint32]
     L.lps_step.argtypes = [vp, C.c_float, C.c_uint32, vp]
+    L.lps_set_tuning.argtypes = [vp, C.POINTER(CTuning)]
+    L.lps_get_tuning.argtypes = [vp, C.POINTER(CTuning)]
+    L.lps_autotune.argtypes = [vp, C.c_float, C.c_uint32, C.POINTER(CTuning)]
     L.lps_get_grid.argtypes = [vp, C.POINTER(CGridInfo), vp, vp, vp, vp, C.c_uint32]
     L.lps_get_fields.argtypes = [vp, vp, vp, vp, vp]
     L.lps_mlp_set_weights.argtypes = [vp, _f32p, _f32p, C.c_float]

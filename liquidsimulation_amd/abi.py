@@ -97,6 +97,15 @@ class CTrainCfg(C.Structure):
     ]
 
 
+class CTuning(C.Structure):
+    """lps_tuning."""
+    _fields_ = [
+        ("use_graphs", C.c_int32),
+        ("lookahead", C.c_uint32),
+        ("ms_per_step", C.c_float),
+    ]
+
+
 class CGridInfo(C.Structure):
     _fields_ = [
         ("bbox_min", C.c_float * 3),

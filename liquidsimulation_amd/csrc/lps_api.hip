@@ -107,6 +107,20 @@ struct lps_ctx {
     float *d_grad = nullptr, *d_m1 = nullptr, *d_m2 = nullptr, *d_loss = nullptr;
     double* d_stats = nullptr;
 
+    // captured step prefixes (grid build, density, forces and corrections
+    // 1..min of a PCISPH step) keyed by state-buffer parity, seeding and
+    // feature capture; valid while `epoch` (bumped by every change of
+    // buffers or parameters) and dt match
+    struct PrefixGraph {
+        hipGraphExec_t exec = nullptr;
+        uint64_t epoch = 0;
+        float dt = 0.f;
+    };
+    PrefixGraph graphs[2][2][2];
+    uint64_t epoch = 1;
+    bool graphs_on = true;
+    uint32_t lookahead = 1;   // corrections kept queued ahead of the one waited for
+
     // profiling
     bool prof = false;
     std::vector<hipEvent_t> ev_pool;
@@ -267,6 +281,7 @@ uint64_t scene_cell_bound(const lps_ctx* c, const float* x, uint32_t n, const st
 
 int ensure_cells(lps_ctx* c, uint32_t cap) {
     if (cap <= c->cell_cap && c->cell_lb) return LPS_OK;
+    c->epoch++;
     int st;
     if ((st = dalloc(&c->cell_lb, (size_t)cap + 1))) return st;
     if ((st = dalloc(&c->run_end, (size_t)cap + 1))) return st;
@@ -280,6 +295,7 @@ int ensure_cells(lps_ctx* c, uint32_t cap) {
 
 int ensure_particles(lps_ctx* c, uint32_t n) {
     if (n <= c->cap_n && c->pos[0]) return LPS_OK;
+    c->epoch++;
     int st;
     for (int b = 0; b < 2; b++) {
         if ((st = dalloc(&c->pos[b], (size_t)n + LPS_STENCIL_PAD))) return st;
@@ -441,6 +457,7 @@ bool capture_features(const lps_ctx* c, bool seeded) {
 
 int ensure_feat(lps_ctx* c) {
     int st;
+    if (!c->feat || !c->seed) c->epoch++;
     if (!c->feat && (st = dalloc(&c->feat, (size_t)c->cap_n * LPS_MLP_IN))) return st;
     if (!c->seed && (st = dalloc(&c->seed, (size_t)c->cap_n))) return st;
     return LPS_OK;
@@ -450,19 +467,99 @@ lps_dev::MlpArgs mlp_args(const lps_ctx* c) {
     return lps_dev::MlpArgs{c->d_mlp, c->d_mlp + LPS_MLP_PARAMS, c->mlp_tscale};
 }
 
+LoopRule loop_rule(const lps_ctx* c) {
+    const uint32_t kmin = (uint32_t)std::max(1, c->params.min_iterations);
+    const uint32_t kmax = (uint32_t)std::max(c->params.max_iterations, (int32_t)kmin);
+    return LoopRule{kmin, kmax, (uint32_t)std::max(0, c->params.stall_iterations),
+                    c->params.max_density_error_ratio};
+}
+
+// One PCISPH correction kk: predicted density (+ loop rule), the seed after
+// the first one, pressure force and next prediction.  `event` records the
+// host-visible completion of the density pass (not inside captured graphs).
+void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const LoopRule& rule, float* feat,
+                  bool seeded, float delta, bool event) {
+    const hipStream_t s = c->stream;
+    {
+        Timed t(c, "pci_density");
+        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule, feat,
+                                    c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
+    }
+    if (event) (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
+    if (kk == 1u && seeded) {
+        // the first correction's pressure update is replaced by the seed
+        Timed t(c, "mlp_seed");
+        lps_dev::launch_mlp_seed(s, mlp_args(c), c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
+                                 delta * c->params.rest_density,
+                                 c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c));
+    }
+    {
+        Timed t(c, "pci_pressure");
+        lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp, c->xpred[kk & 1u],
+                                     kk);
+    }
+}
+
+// The fixed part of a PCISPH step: grid build (the state flips to the other
+// buffer), density, forces + first prediction, corrections 1..min.
+void enqueue_prefix(lps_ctx* c, float dt, float delta, bool seeded, float* feat, bool events) {
+    const hipStream_t s = c->stream;
+    enqueue_grid(c);
+    const SimConst k = make_const(c, dt, delta);
+    const lps_dev::PhysArgs a = phys_args(c, k);
+    {
+        Timed t(c, "density");
+        lps_dev::launch_density(s, a, c->press, c->q);
+    }
+    {
+        Timed t(c, "forces");
+        lps_dev::launch_forces(s, a, c->xpred[0], c->pprev[c->cur], feat);
+    }
+    const LoopRule rule = loop_rule(c);
+    for (uint32_t kk = 1; kk <= rule.min_iter; kk++) enqueue_iter(c, a, kk, rule, feat, seeded, delta, events);
+}
+
+// The prefix as a captured HIP graph, replayed when nothing it depends on
+// changed.  Returns false when graphs are unavailable (nothing enqueued).
+bool launch_prefix_graph(lps_ctx* c, float dt, float delta, bool seeded, bool feat_on, float* feat) {
+    const int cur0 = c->cur;
+    lps_ctx::PrefixGraph& pg = c->graphs[cur0][seeded ? 1 : 0][feat_on ? 1 : 0];
+    if (!pg.exec || pg.epoch != c->epoch || pg.dt != dt) {
+        if (pg.exec) (void)hipGraphExecDestroy(pg.exec);
+        pg.exec = nullptr;
+        if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+        enqueue_prefix(c, dt, delta, seeded, feat, false);
+        c->cur = cur0;  // the capture enqueued nothing; the launch below flips it
+        hipGraph_t graph = nullptr;
+        const bool ok = hipStreamEndCapture(c->stream, &graph) == hipSuccess && graph &&
+                        hipGraphInstantiate(&pg.exec, graph, nullptr, nullptr, 0) == hipSuccess;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (!ok) {
+            pg.exec = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+        pg.epoch = c->epoch;
+        pg.dt = dt;
+    }
+    if (hipGraphLaunch(pg.exec, c->stream) != hipSuccess) return false;
+    c->cur = 1 - cur0;
+    return true;
+}
+
 StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     StepResult res;
     const hipStream_t s = c->stream;
     const bool pci = c->params.method == LPS_METHOD_PCISPH;
     const float delta = pci ? lps_compute_delta(&c->params, &c->der, dt) : 0.f;
-    enqueue_grid(c);
-    const SimConst k = make_const(c, dt, delta);
-    lps_dev::PhysArgs a = phys_args(c, k);
-    {
-        Timed t(c, "density");
-        lps_dev::launch_density(s, a, c->press, c->q);
-    }
     if (!pci) {
+        enqueue_grid(c);
+        const SimConst k = make_const(c, dt, delta);
+        lps_dev::PhysArgs a = phys_args(c, k);
+        {
+            Timed t(c, "density");
+            lps_dev::launch_density(s, a, c->press, c->q);
+        }
         {
             Timed t(c, "forces");
             lps_dev::launch_forces(s, a, nullptr, nullptr, nullptr);
@@ -480,42 +577,24 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
         return res;
     }
     float* feat = feat_on ? c->feat : nullptr;
-    {
-        Timed t(c, "forces");
-        lps_dev::launch_forces(s, a, c->xpred[0], c->pprev[c->cur], feat);
+    const LoopRule rule = loop_rule(c);
+    const uint32_t kmin = rule.min_iter, kmax = rule.max_iter;
+    bool graphed = c->graphs_on && !c->prof && launch_prefix_graph(c, dt, delta, seeded, feat_on, feat);
+    if (graphed) {
+        (void)hipEventRecord(c->iter_ev[kmin % LPS_MAX_ITER_SLOTS], s);
+    } else {
+        if (c->graphs_on && !c->prof) c->graphs_on = false;  // capture or launch failed: stay eager
+        enqueue_prefix(c, dt, delta, seeded, feat, true);
     }
-    const uint32_t kmin = (uint32_t)std::max(1, c->params.min_iterations);
-    const uint32_t kmax = (uint32_t)std::max(c->params.max_iterations, (int32_t)kmin);
-    const LoopRule rule{kmin, kmax, (uint32_t)std::max(0, c->params.stall_iterations),
-                        c->params.max_density_error_ratio};
-    auto enqueue_iter = [&](uint32_t kk) {
-        {
-            Timed t(c, "pci_density");
-            lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule, feat,
-                                        c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
-        }
-        (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
-        if (kk == 1u && seeded) {
-            // the first correction's pressure update is replaced by the seed
-            Timed t(c, "mlp_seed");
-            lps_dev::launch_mlp_seed(s, mlp_args(c), c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
-                                     delta * c->params.rest_density,
-                                     c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c));
-        }
-        {
-            Timed t(c, "pci_pressure");
-            lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp,
-                                         c->xpred[kk & 1u], kk);
-        }
-    };
-    // Corrections 1..min always run; afterwards one correction stays queued
-    // ahead of the result being waited for.  Queued corrections past the stop
-    // return at once on the device.
-    for (uint32_t kk = 1; kk <= kmin; kk++) enqueue_iter(kk);
+    const SimConst k = make_const(c, dt, delta);
+    lps_dev::PhysArgs a = phys_args(c, k);
+    // Corrections beyond min: `lookahead` corrections stay queued ahead of
+    // the result being waited for; queued corrections past the stop return
+    // at once on the device.
     uint32_t k_enq = kmin, k_known = 0;
     while (true) {
-        if (k_enq < kmax) enqueue_iter(++k_enq);
         const uint32_t kw = std::max(k_known + 1u, kmin);
+        while (k_enq < kmax && k_enq < kw + c->lookahead) enqueue_iter(c, a, ++k_enq, rule, feat, seeded, delta, true);
         (void)hipEventSynchronize(c->iter_ev[kw % LPS_MAX_ITER_SLOTS]);
         volatile uint32_t* f = c->flags + 4 * (kw % LPS_MAX_ITER_SLOTS);
         const uint32_t state = f[0];
@@ -675,6 +754,10 @@ int lps_destroy(lps_ctx* c) {
     for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++)
         if (c->iter_ev[k]) (void)hipEventDestroy(c->iter_ev[k]);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    for (auto& a : c->graphs)
+        for (auto& b : a)
+            for (auto& g : b)
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return LPS_OK;
@@ -685,6 +768,7 @@ int lps_set_params(lps_ctx* c, const lps_params* p) {
     int st = check_params(p);
     if (st) return st;
     c->params = *p;
+    c->epoch++;
     lps_compute_derived(&c->params, &c->der);
     st = upload_tables(c);
     return st ? st : upload_scenes(c);
@@ -838,6 +922,7 @@ int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const f
     c->n = n;
     c->n_real = (uint32_t)total;
     c->have_step = false;
+    c->epoch++;
     if ((st = upload_scenes(c))) return st;
     lps_dev::launch_bbox(c->stream, c->pos[0], n, scenes(c), c->bbox_part);
     HIPC(hipStreamSynchronize(c->stream));
@@ -1001,6 +1086,122 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
     return LPS_OK;
 }
 
+int lps_set_tuning(lps_ctx* c, const lps_tuning* t) {
+    if (!c || !t || (t->use_graphs != 0 && t->use_graphs != 1) || t->lookahead < 1 || t->lookahead > 4)
+        return LPS_ERR_INVALID;
+    c->graphs_on = t->use_graphs != 0;
+    c->lookahead = t->lookahead;
+    return LPS_OK;
+}
+
+int lps_get_tuning(lps_ctx* c, lps_tuning* t) {
+    if (!c || !t) return LPS_ERR_INVALID;
+    t->use_graphs = c->graphs_on ? 1 : 0;
+    t->lookahead = c->lookahead;
+    t->ms_per_step = 0.f;
+    return LPS_OK;
+}
+
+int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
+    if (!c || !(dt > 0.f) || steps == 0 || steps > 1000) return LPS_ERR_INVALID;
+    if (c->n_real == 0) return LPS_ERR_STATE;
+    if (c->prof) return LPS_ERR_STATE;
+    const hipStream_t s = c->stream;
+    const uint32_t n = c->n, cur = (uint32_t)c->cur;
+    const size_t nparts = (size_t)LPS_BBOX_PART * lps_dev::bbox_parts(n);
+    // snapshot of everything the next step reads
+    float4 *sp = nullptr, *sv = nullptr, *sh = nullptr;
+    uint8_t* sf = nullptr;
+    uint32_t *si = nullptr, *sb = nullptr;
+    float* spp = nullptr;
+    int st = LPS_OK;
+    if ((st = dalloc(&sp, n)) || (st = dalloc(&sv, n)) || (st = dalloc(&sh, n)) || (st = dalloc(&sf, n)) ||
+        (st = dalloc(&si, n)) || (st = dalloc(&spp, n)) || (st = dalloc(&sb, nparts))) {
+        dfree(sp); dfree(sv); dfree(sh); dfree(sf); dfree(si); dfree(spp); dfree(sb);
+        return st;
+    }
+    auto copy_all = [&](bool save) -> hipError_t {
+        const int b = c->cur;
+        hipError_t e = hipSuccess;
+        auto cp = [&](void* dst, void* src, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+        };
+        if (save) {
+            cp(sp, c->pos[b], sizeof(float4) * n); cp(sv, c->vel[b], sizeof(float4) * n);
+            cp(sh, c->vh[b], sizeof(float4) * n); cp(sf, c->first[b], n);
+            cp(si, c->id[b], sizeof(uint32_t) * n); cp(spp, c->pprev[b], sizeof(float) * n);
+            cp(sb, c->bbox_part, sizeof(uint32_t) * nparts);
+        } else {
+            cp(c->pos[b], sp, sizeof(float4) * n); cp(c->vel[b], sv, sizeof(float4) * n);
+            cp(c->vh[b], sh, sizeof(float4) * n); cp(c->first[b], sf, n);
+            cp(c->id[b], si, sizeof(uint32_t) * n); cp(c->pprev[b], spp, sizeof(float) * n);
+            cp(c->bbox_part, sb, sizeof(uint32_t) * nparts);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        return e;
+    };
+    const bool g0 = c->graphs_on;
+    const uint32_t l0 = c->lookahead;
+    const uint32_t coll0 = c->coll_stride;
+    c->coll_stride = 0;  // no training rows from trial steps
+    const double ema_s = c->ema_seeded, ema_p = c->ema_plain;
+    const bool ws = c->ws_on;
+    const uint64_t wss = c->ws_steps, steps0 = c->steps;
+    if (copy_all(true) != hipSuccess) st = LPS_ERR_HIP;
+    struct Cand { bool g; uint32_t l; float ms; };
+    std::vector<Cand> cands;
+    for (int g = 0; g < 2; g++)
+        for (uint32_t l = 1; l <= 3; l++) cands.push_back({g == 1, l, 0.f});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = LPS_ERR_HIP;
+    for (Cand& cd : cands) {
+        if (st) break;
+        c->graphs_on = cd.g;
+        c->lookahead = cd.l;
+        // one untimed step (graph capture), then the timed ones, each from the snapshot
+        for (int pass = 0; pass < 2 && !st; pass++) {
+            c->cur = (int)cur;
+            c->ema_seeded = ema_s; c->ema_plain = ema_p; c->ws_on = ws; c->ws_steps = wss;
+            if (copy_all(false) != hipSuccess) { st = LPS_ERR_HIP; break; }
+            if (pass == 1) (void)hipEventRecord(e0, s);
+            st = lps_step(c, dt, pass == 0 ? 1u : steps, nullptr);
+            if (pass == 1) {
+                (void)hipEventRecord(e1, s);
+                (void)hipEventSynchronize(e1);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                cd.ms = ms / (float)steps;
+            }
+        }
+        if (cd.g && !c->graphs_on) cd.ms = INFINITY;  // graphs unavailable on this device
+    }
+    // restore the state and the scheduling bookkeeping; keep the fastest
+    c->cur = (int)cur;
+    c->ema_seeded = ema_s; c->ema_plain = ema_p; c->ws_on = ws; c->ws_steps = wss; c->steps = steps0;
+    c->coll_stride = coll0;
+    if (copy_all(false) != hipSuccess && !st) st = LPS_ERR_HIP;
+    c->have_step = false;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(sp); dfree(sv); dfree(sh); dfree(sf); dfree(si); dfree(spp); dfree(sb);
+    if (st) {
+        c->graphs_on = g0;
+        c->lookahead = l0;
+        return st;
+    }
+    const Cand* best = &cands[0];
+    for (const Cand& cd : cands)
+        if (cd.ms < best->ms) best = &cd;
+    c->graphs_on = best->g;
+    c->lookahead = best->l;
+    if (chosen) {
+        chosen->use_graphs = best->g ? 1 : 0;
+        chosen->lookahead = best->l;
+        chosen->ms_per_step = best->ms;
+    }
+    return LPS_OK;
+}
+
 int lps_get_grid(lps_ctx* c, lps_grid_info* info, uint32_t* keys, uint32_t* perm, uint32_t* cell_start,
                  uint32_t* cell_end, uint32_t cap) {
     if (!c || !c->have_step) return LPS_ERR_STATE;
@@ -1107,6 +1308,7 @@ int lps_mlp_set_weights(lps_ctx* c, const float* params, const float* feature_sc
                         hipMemcpyHostToDevice, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
     c->mlp_loaded = true;
+    c->epoch++;
     c->ema_seeded = c->ema_plain = -1.0;
     c->ws_on = true;
     c->ws_steps = 0;
@@ -1291,6 +1493,7 @@ int lps_mlp_train(lps_ctx* c, const lps_mlp_train_cfg* cfg, float* epoch_loss) {
             return LPS_ERR_NONFINITE;
         }
     c->mlp_loaded = true;
+    c->epoch++;
     c->ema_seeded = c->ema_plain = -1.0;
     c->ws_on = true;
     c->ws_steps = 0;

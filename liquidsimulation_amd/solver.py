@@ -17,7 +17,7 @@ from typing import List, Optional
 import numpy as np
 
 from . import _lib
-from .abi import (CBox, CGridInfo, CParams, CPlane, CSphere, CStepStats, CTrainCfg, Box, Colliders, Params, Plane,
+from .abi import (CBox, CGridInfo, CParams, CPlane, CSphere, CStepStats, CTrainCfg, CTuning, Box, Colliders, Params, Plane,
                   Sphere, LPS_TABLE_SIZE, LPS_MLP_IN, LPS_MLP_PARAMS, METHOD_PCISPH, METHOD_SPH)
 from ._lib import check, lib, ptr
 
@@ -259,6 +259,24 @@ class SPHSolver:
         d = C.c_float()
         check(lib().lps_get_delta(self._ctx, float(dt), C.byref(d)), "lps_get_delta")
         return d.value
+
+    # ---- launch tuning
+    def set_tuning(self, use_graphs: bool, lookahead: int = 1):
+        t = CTuning(use_graphs=int(bool(use_graphs)), lookahead=int(lookahead), ms_per_step=0.0)
+        check(lib().lps_set_tuning(self._ctx, C.byref(t)), "lps_set_tuning")
+
+    def get_tuning(self):
+        t = CTuning()
+        check(lib().lps_get_tuning(self._ctx, C.byref(t)), "lps_get_tuning")
+        return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead}
+
+    def autotune(self, dt: float, steps: int = 10):
+        """Time every launch setting for `steps` steps from the current state,
+        restore the state and keep the fastest setting."""
+        self._flush_pending()
+        t = CTuning()
+        check(lib().lps_autotune(self._ctx, float(dt), int(steps), C.byref(t)), "lps_autotune")
+        return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead, "ms_per_step": t.ms_per_step}
 
     # ---- timing
     def profile(self, on: bool = True):
