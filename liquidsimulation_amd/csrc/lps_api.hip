@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -82,7 +83,6 @@ struct lps_ctx {
     // host
     uint32_t* flags = nullptr;  // pinned, coherent: 4 words per iteration slot
     DevGrid* h_grid = nullptr;  // pinned copy
-    hipEvent_t iter_ev[LPS_MAX_ITER_SLOTS] = {};
     int last_fp_valid = 0;      // PCISPH fields available
 
     // learned warm start: fp32 master weights, feature normalisation, target scale
@@ -474,18 +474,39 @@ LoopRule loop_rule(const lps_ctx* c) {
                     c->params.max_density_error_ratio};
 }
 
+// Host flag slot of correction kk: {state, corrections, error bits,
+// converged}, written by the device (word 0 last).  The host marks it
+// pending before the correction is enqueued and polls word 0.
+constexpr uint32_t FLAG_PENDING = 0xFFFFFFFFu;
+volatile uint32_t* flag_slot(lps_ctx* c, uint32_t kk) { return c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS); }
+
+// Waits until the device has written slot kk; false after ~10 s without it.
+bool wait_flag(lps_ctx* c, uint32_t kk) {
+    volatile uint32_t* f = flag_slot(c, kk);
+    for (uint64_t spin = 0; f[0] == FLAG_PENDING; spin++) {
+        if ((spin & 1023u) == 1023u) {
+            // the correction may still be queued behind a long kernel
+            if (hipStreamQuery(c->stream) == hipSuccess && f[0] == FLAG_PENDING) return false;
+            if (spin > (1ull << 32)) return false;
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return true;
+}
+
 // One PCISPH correction kk: predicted density (+ loop rule), the seed after
-// the first one, pressure force and next prediction.  `event` records the
-// host-visible completion of the density pass (not inside captured graphs).
+// the first one, pressure force and next prediction.  `mark` sets the host
+// flag slot pending (outside captured graphs; the caller marks those).
 void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const LoopRule& rule, float* feat,
-                  bool seeded, float delta, bool event) {
+                  bool seeded, float delta, bool mark) {
     const hipStream_t s = c->stream;
+    if (mark) flag_slot(c, kk)[0] = FLAG_PENDING;
     {
         Timed t(c, "pci_density");
         lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule, feat,
                                     c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
     }
-    if (event) (void)hipEventRecord(c->iter_ev[kk % LPS_MAX_ITER_SLOTS], s);
     if (kk == 1u && seeded) {
         // the first correction's pressure update is replaced by the seed
         Timed t(c, "mlp_seed");
@@ -579,10 +600,9 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     float* feat = feat_on ? c->feat : nullptr;
     const LoopRule rule = loop_rule(c);
     const uint32_t kmin = rule.min_iter, kmax = rule.max_iter;
+    for (uint32_t kk = 1; kk <= kmin; kk++) flag_slot(c, kk)[0] = FLAG_PENDING;
     bool graphed = c->graphs_on && !c->prof && launch_prefix_graph(c, dt, delta, seeded, feat_on, feat);
-    if (graphed) {
-        (void)hipEventRecord(c->iter_ev[kmin % LPS_MAX_ITER_SLOTS], s);
-    } else {
+    if (!graphed) {
         if (c->graphs_on && !c->prof) c->graphs_on = false;  // capture or launch failed: stay eager
         enqueue_prefix(c, dt, delta, seeded, feat, true);
     }
@@ -595,8 +615,12 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     while (true) {
         const uint32_t kw = std::max(k_known + 1u, kmin);
         while (k_enq < kmax && k_enq < kw + c->lookahead) enqueue_iter(c, a, ++k_enq, rule, feat, seeded, delta, true);
-        (void)hipEventSynchronize(c->iter_ev[kw % LPS_MAX_ITER_SLOTS]);
-        volatile uint32_t* f = c->flags + 4 * (kw % LPS_MAX_ITER_SLOTS);
+        if (!wait_flag(c, kw)) {
+            res.ok = false;  // the device never answered: reported as a HIP error
+            res.iterations = -1;
+            break;
+        }
+        volatile uint32_t* f = flag_slot(c, kw);
         const uint32_t state = f[0];
         res.iterations = (int32_t)f[1];
         res.err_bits = f[2];
@@ -731,8 +755,6 @@ int lps_create(const lps_params* params, lps_ctx** out) {
               hipHostMalloc((void**)&c->h_grid, sizeof(DevGrid), hipHostMallocDefault) == hipSuccess &&
               hipMemsetAsync(c->grid, 0, sizeof(DevGrid), c->stream) == hipSuccess &&
               upload_tables(c) == LPS_OK;
-    for (int k = 0; ok && k < LPS_MAX_ITER_SLOTS; k++)
-        ok = hipEventCreateWithFlags(&c->iter_ev[k], hipEventDisableTiming) == hipSuccess;
     if (ok) {
         ok = hipMalloc((void**)&c->boxes, sizeof(lps_box) * LPS_MAX_COLLIDERS) == hipSuccess &&
              hipMalloc((void**)&c->spheres, sizeof(lps_sphere) * LPS_MAX_COLLIDERS) == hipSuccess &&
@@ -751,8 +773,6 @@ int lps_destroy(lps_ctx* c) {
     if (!c) return LPS_ERR_INVALID;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_all(c);
-    for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++)
-        if (c->iter_ev[k]) (void)hipEventDestroy(c->iter_ev[k]);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     for (auto& a : c->graphs)
         for (auto& b : a)
@@ -1073,6 +1093,7 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
             if (stats && completed < nsteps) stats[completed].status = (int32_t)status;
             return (int)status;
         }
+        if (pci && last.iterations < 0) return LPS_ERR_HIP;
         if (pci && warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
     }
     if (pci) {

@@ -991,11 +991,16 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     uint32_t* __restrict__ host_flag, lps_dev::Scenes S, uint32_t nlogical) {
     const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
     if (step_dead(g) || g->running == 0u) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-            host_flag[0] = step_dead(g) ? 2u : 1u;
+        // A correction queued past the stop answers nothing: the host never
+        // waits on it, and it may run after the host has marked the same slot
+        // for the next step.  A dead step is answered (the host may be waiting
+        // on any correction of it) and the host drains the stream after it.
+        if (step_dead(g) && blockIdx.x == 0 && threadIdx.x == 0) {
             host_flag[1] = g->iterations;
             host_flag[2] = g->final_err;
             host_flag[3] = g->converged;
+            __threadfence_system();  // the host polls word 0, then reads the others
+            host_flag[0] = 2u;
         }
         return;
     }
@@ -1132,10 +1137,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     g->iterations = k;
     g->final_err = s_err;
     g->converged = s_conv;
-    host_flag[0] = state;
     host_flag[1] = k;
     host_flag[2] = s_err;
     host_flag[3] = s_conv;
+    __threadfence_system();  // the host polls word 0, then reads the others
+    host_flag[0] = state;
 }
 
 // pciCalcPressureForce (SPHKernels.cl:806-878) followed by the next

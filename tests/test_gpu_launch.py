@@ -19,7 +19,7 @@ def _scene():
     return sc
 
 
-def _run(sc, steps, use_graphs, lookahead, warm=None, autotune=False):
+def _run(sc, steps, use_graphs, lookahead, warm=None, autotune=False, one_call=False):
     s = PCISPHSolver(sc.params.replace(warm_start=WARM_LEARNED) if warm is not None else sc.params)
     if warm is not None:
         s.set_warm_start_weights(*warm)
@@ -28,10 +28,9 @@ def _run(sc, steps, use_graphs, lookahead, warm=None, autotune=False):
     s.set_tuning(use_graphs, lookahead)
     tuned = s.autotune(sc.dt, 3) if autotune else None
     its, used = [], []
-    for _ in range(steps):
-        st = s.update(sc.dt, 1)
-        its.append(st[0].iterations)
-        used.append(st[0].warm_start_used)
+    for st in (s.update(sc.dt, steps) if one_call else [s.update(sc.dt, 1)[0] for _ in range(steps)]):
+        its.append(st.iterations)
+        used.append(st.warm_start_used)
     p = s.get_particles()
     s.close()
     return p, its, used, tuned
@@ -64,3 +63,18 @@ def test_autotune_leaves_the_trajectory_unchanged(gpu):
     assert its == its_ref
     for k in ("x", "v", "vhalf"):
         assert ulp_diff(out[k], ref[k]) == 0, k
+
+
+@pytest.mark.parametrize("lookahead", [1, 3])
+def test_many_steps_in_one_call_match_step_by_step(gpu, lookahead):
+    """Steps issued back to back in one call (no host synchronisation between
+    them) leave queued lookahead corrections of one step running while the
+    host already polls the next step's corrections; the answers must still be
+    the current step's."""
+    sc = _scene()
+    ref, its_ref, _, _ = _run(sc, 24, False, 1)
+    for g in (False, True):
+        out, its, _, _ = _run(sc, 24, g, lookahead, one_call=True)
+        assert its == its_ref, (g, lookahead, its, its_ref)
+        for k in ("x", "v", "vhalf"):
+            assert ulp_diff(out[k], ref[k]) == 0, (g, lookahead, k)
