@@ -120,6 +120,11 @@ static inline __host__ __device__ float o2f(uint32_t u) {
     return c.f;
 }
 
+// Accept masks recorded per particle by a neighbour pass: one 32-bit mask
+// per row of the 27-cell stencil (9 rows), covering a row's first 32
+// candidates; a replaying pass filters any further candidates live.
+#define LPS_MC_ROWS 9
+
 // Position arrays scanned by the stencils carry this many padding entries:
 // the candidate filter reads whole groups of eight past a row's end.
 #define LPS_STENCIL_PAD 8

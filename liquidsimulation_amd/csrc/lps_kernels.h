@@ -23,6 +23,19 @@ struct Scenes {
     uint32_t* act_list;      // [2][nblocks] active-block lists of the correction kernels
 };
 
+// Candidate masks recorded by one neighbour pass for the next pass over the
+// same positions and cells (k_density -> k_forces / k_pressure_sph,
+// k_pci_density -> k_pci_pressure of the same correction): per slot, the
+// accept mask of the first 32 candidates of each of the 9 rows of its
+// 27-cell stencil (bit t: candidate row_start + t), row-major (row r of slot
+// i at mask[r * stride + i], so a wave's lanes touch consecutive words), and
+// a flag that the slot was recorded.
+struct NbMasks {
+    uint32_t* mask;    // [LPS_MC_ROWS][stride]
+    uint32_t* ok;      // [stride] 1: recorded by the last recording pass
+    uint32_t stride;   // slots (LPS_MC_ROWS * stride < 2^32)
+};
+
 struct ReorderArgs {
     const uint32_t *keysA, *valsA, *keysB, *valsB;
     uint32_t n;
@@ -55,6 +68,7 @@ struct PhysArgs {
     const lps_box* boxes;
     const lps_sphere* spheres;
     const lps_plane* planes;
+    NbMasks mk;
     SimConst c;
 };
 

@@ -152,37 +152,94 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
 // pass 1 runs a cheap filter over every candidate (eight independent loads in
 // flight; squared distance with fused multiply-adds against s_relax, which
 // bounds its rounding difference to the exact form) and records a per-lane
-// accept mask; pass 2 visits the kept candidates in ascending order, four
-// loads at a time (position and pre(j)), forms the exact squared distance s2
-// (the kernels' unfused expression) and calls fn(j, x_j, s2, pre(j)) when
-// s2 <= s_max, i.e. when d = sqrtf(s2) < h.  Accepted set and visiting order
-// are those of for_neighbors, so sums are bit-identical.  Candidates are
-// addressed by 32-bit byte offsets from xs (particle counts stay below 2^28);
-// pass 1 reads up to 7 entries past a row's end, so xs carries
-// LPS_STENCIL_PAD entries of padding (masked out, never used).
+// accept mask; pass 2 (visit_window) visits the kept candidates in ascending
+// order, four loads at a time (position and pre(j)), forms the exact squared
+// distance s2 (the kernels' unfused expression) and calls fn(j, x_j, s2,
+// pre(j)) when s2 <= s_max, i.e. when d = sqrtf(s2) < h.  Accepted set and
+// visiting order are those of for_neighbors, so sums are bit-identical.
+// Candidates are addressed by 32-bit byte offsets from xs (particle counts
+// stay below 2^28); pass 1 reads up to 7 entries past a row's end, so xs
+// carries LPS_STENCIL_PAD entries of padding (masked out, never used).
+//
+// Recorded masks (NbMasks): MC_RECORD also stores the accept mask of the
+// first window of each in-grid row (0 for an empty row); MC_REPLAY, in a
+// later pass over the same positions and anchor cells, runs pass 2 on the
+// recorded masks (those pass 1 would compute again) and filters live only
+// the windows past the first of a row longer than 32 candidates, or the whole
+// stencil when the slot was not recorded.  Every mode accepts the same pairs
+// in the same order.
+enum : int { MC_LIVE = 0, MC_RECORD = 1, MC_REPLAY = 2 };
+
+template <class Acc, class Pre, class F>
+__device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px,
+                                             float py, float pz, float s_max, Acc& accept_extra, Pre& pre, F& fn) {
+    while (mask) {
+        uint32_t kk[4];
+        uint32_t cnt = 1u;
+        kk[0] = (uint32_t)__ffs((int)mask) - 1u;
+        mask &= mask - 1u;
+#pragma unroll
+        for (int t = 1; t < 4; t++) {
+            kk[t] = kk[0];
+            if (mask) {
+                kk[t] = (uint32_t)__ffs((int)mask) - 1u;
+                mask &= mask - 1u;
+                cnt++;
+            }
+        }
+        float4 q[4];
+        decltype(pre(0u)) pv[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            q[t] = *reinterpret_cast<const float4*>(xb + ((w0 + kk[t]) << 4));
+            pv[t] = pre(w0 + kk[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if ((uint32_t)t < cnt) {
+                const uint32_t j = w0 + kk[t];
+                const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
+                const float s2 = dx * dx + dy * dy + dz * dz;
+                if (s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t]);
+            }
+        }
+    }
+}
+
 template <class Acc, class Pre, class F>
 __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const uint32_t* __restrict__ lb, float ax,
                                                      float ay, float az, const float4* __restrict__ xs, float px,
                                                      float py, float pz, float s_max, float s_relax,
+                                                     const lps_dev::NbMasks& M, uint32_t i, int mode,
                                                      Acc&& accept_extra, Pre&& pre, F&& fn) {
+    const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
     const int xg = (int)truncf((ax + G.off0) / G.gs);
     const int yg = (int)truncf((ay + G.off1) / G.gs);
     const int zg = (int)truncf((az + G.off2) / G.gs);
     const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
-    const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
-    auto at = [&](uint32_t j) { return *reinterpret_cast<const float4*>(xb + (j << 4)); };
+    const bool replay = mode == MC_REPLAY && M.ok[i] != 0u;
+    const bool record = mode == MC_RECORD;
     auto keep = [&](const float4& q) {
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) <= s_relax;
     };
-    for (int z = zg - 1; z <= zg + 1; z++) {
+    for (uint32_t rz = 0; rz < 3u; rz++) {
+        const int z = zg - 1 + (int)rz;
         if (z < 0 || z >= G.gz) continue;
-        for (int y = yg - 1; y <= yg + 1; y++) {
+        for (uint32_t ry = 0; ry < 3u; ry++) {
+            const int y = yg - 1 + (int)ry;
             if (y < 0 || y >= G.gy) continue;
+            uint32_t* __restrict__ row = M.mask + (3u * rz + ry) * M.stride;
             const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
             const uint32_t b = lb[base + (uint32_t)x0];
             const uint32_t e = lb[base + (uint32_t)x1 + 1u];
-            for (uint32_t w0 = b; w0 < e; w0 += 32u) {
+            uint32_t w0 = b;
+            if (replay) {
+                visit_window(xb, b, row[i], px, py, pz, s_max, accept_extra, pre, fn);
+                w0 += 32u;
+            }
+            uint32_t first = 0u;
+            for (; w0 < e; w0 += 32u) {
                 const uint32_t n = min(32u, e - w0);
                 uint32_t mask = 0u;
                 for (uint32_t u = 0; u < n; u += 8u) {
@@ -196,40 +253,13 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
                     mask |= m8 << u;
                 }
                 mask &= n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u;
-                while (mask) {
-                    uint32_t kk[4];
-                    uint32_t cnt = 1u;
-                    kk[0] = (uint32_t)__ffs((int)mask) - 1u;
-                    mask &= mask - 1u;
-#pragma unroll
-                    for (int t = 1; t < 4; t++) {
-                        kk[t] = kk[0];
-                        if (mask) {
-                            kk[t] = (uint32_t)__ffs((int)mask) - 1u;
-                            mask &= mask - 1u;
-                            cnt++;
-                        }
-                    }
-                    float4 q[4];
-                    decltype(pre(0u)) pv[4];
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        q[t] = at(w0 + kk[t]);
-                        pv[t] = pre(w0 + kk[t]);
-                    }
-#pragma unroll
-                    for (int t = 0; t < 4; t++) {
-                        if ((uint32_t)t < cnt) {
-                            const uint32_t j = w0 + kk[t];
-                            const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
-                            const float s2 = dx * dx + dy * dy + dz * dz;
-                            if (s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t]);
-                        }
-                    }
-                }
+                if (w0 == b) first = mask;
+                visit_window(xb, w0, mask, px, py, pz, s_max, accept_extra, pre, fn);
             }
+            if (record) row[i] = first;
         }
     }
+    if (record) M.ok[i] = 1u;
 }
 
 // Lookup index min(trunc(sqrtf(s2) / div), 1000) of a squared distance
@@ -816,7 +846,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
                                                        lps_dev::Scenes S, const uint32_t* __restrict__ cell_lb,
                                                        SimConst c, const float* __restrict__ tables,
                                                        float4* __restrict__ posrho, float* __restrict__ press,
-                                                       float* __restrict__ q, uint32_t nlogical) {
+                                                       float* __restrict__ q, lps_dev::NbMasks mk,
+                                                       uint32_t nlogical) {
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
@@ -830,7 +861,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     const GridView G = grid_view(g, S.grid + B.s);
     const float4 xi = pos[i];
     float sum = 0.f;
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                          [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                          [&](uint32_t, const float4&, float s2, int) { sum += sW[table_index(s2, sT, c.inv_div)]; });
     const float rho = c.mass * sum;
@@ -850,7 +881,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     const uint8_t* __restrict__ first, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
     const lps_plane* __restrict__ planes0, float4* __restrict__ force, float4* __restrict__ xpred,
-    const float* __restrict__ pprev, float* __restrict__ feat, lps_dev::Scenes S, uint32_t nlogical) {
+    const float* __restrict__ pprev, float* __restrict__ feat, lps_dev::Scenes S, lps_dev::NbMasks mk,
+    uint32_t nlogical) {
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
@@ -874,8 +906,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
     float nb = 0.f, dv = 0.f;  // warm-start features: neighbour count, velocity divergence
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
-                         [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
+    // the density pass recorded this stencil's masks at the same positions
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                         MC_REPLAY, [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
                          [&](uint32_t j, const float4& xj, float s2, const float4& vj) {
         const uint32_t idx = table_index(s2, sT, c.inv_div);
         const float rj = xj.w;
@@ -939,7 +972,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
                                                             const uint32_t* __restrict__ cell_lb, SimConst c,
                                                             const float* __restrict__ tables,
                                                             float4* __restrict__ force, lps_dev::Scenes S,
-                                                            uint32_t nlogical) {
+                                                            lps_dev::NbMasks mk, uint32_t nlogical) {
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
@@ -954,8 +987,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const float4 xi = posrho[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
-                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                         MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
                          [&](uint32_t j, const float4& xj, float s2, float qj) {
         const float d = sqrtf(s2);
         float ux, uy, uz;
@@ -988,7 +1021,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
     float* __restrict__ q, uint32_t k, LoopRule rule, float* __restrict__ feat, const float4* __restrict__ posrho,
-    uint32_t* __restrict__ host_flag, lps_dev::Scenes S, uint32_t nlogical) {
+    uint32_t* __restrict__ host_flag, lps_dev::Scenes S, lps_dev::NbMasks mk, uint32_t nlogical) {
     const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
     if (step_dead(g) || g->running == 0u) {
         // A correction queued past the stop answers nothing: the host never
@@ -1024,8 +1057,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         const float4 xa = pos[i];
         const float4 xi = xpred[i];
         float sum = 0.f;
-        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
-                             [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+        // recorded for the pressure pass of this correction (same positions and cells)
+        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                             MC_RECORD, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                              [&](uint32_t, const float4&, float s2, int) { sum += sW[table_index(s2, sT, c.inv_div)]; });
         const float rp = c.mass * sum;
         float e = rp - c.rest_density;
@@ -1153,7 +1187,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
     const lps_plane* __restrict__ planes0, float4* __restrict__ xpred_next, uint32_t k, lps_dev::Scenes S,
-    uint32_t nlogical) {
+    lps_dev::NbMasks mk, uint32_t nlogical) {
     if (step_dead(g)) return;
     // the blocks of the scenes that ran this correction's predicted-density pass
     const uint32_t lb = active_block(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS]);
@@ -1176,8 +1210,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4 xi = xpred[i];
     const float tf = q[i];
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax,
-                         [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+    // the predicted-density pass of this correction recorded the masks
+    for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                         MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
                          [&](uint32_t j, const float4& xj, float s2, float qj) {
         const float d = sqrtf(s2);
         float ux, uy, uz;
@@ -1302,19 +1337,21 @@ void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint3
 
 void launch_density(hipStream_t s, const PhysArgs& a, float* press, float* q) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.g, a.sc, a.cell_lb, a.c, a.tables, a.posrho, press, q, nl);
+    k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.g, a.sc, a.cell_lb, a.c, a.tables, a.posrho, press, q, a.mk,
+                                                nl);
 }
 
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = feat ? k_forces<true> : k_forces<false>;
     kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables, a.boxes,
-                                            a.spheres, a.planes, a.force, xpred, pprev, feat, a.sc, nl);
+                                            a.spheres, a.planes, a.force, xpred, pprev, feat, a.sc, a.mk, nl);
 }
 
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    k_pressure_sph<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, a.sc, nl);
+    k_pressure_sph<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, a.sc, a.mk,
+                                                     nl);
 }
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
@@ -1322,7 +1359,7 @@ void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, f
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = (feat && k == 1u) ? k_pci_density<true> : k_pci_density<false>;
     kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, rule,
-                                            feat, a.posrho, host_flag, a.sc, nl);
+                                            feat, a.posrho, host_flag, a.sc, a.mk, nl);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
@@ -1330,7 +1367,7 @@ void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, 
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_pci_pressure<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp,
                                                       a.force, a.posrho, a.vel, a.vh, a.first, a.boxes, a.spheres,
-                                                      a.planes, xpred_next, k, a.sc, nl);
+                                                      a.planes, xpred_next, k, a.sc, a.mk, nl);
 }
 
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
