@@ -155,7 +155,7 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
 // accept mask; pass 2 (visit_window) visits the kept candidates in ascending
 // order, four loads at a time (position and pre(j)), forms the exact squared
 // distance s2 (the kernels' unfused expression) and calls fn(j, x_j, s2,
-// pre(j)) when s2 <= s_max, i.e. when d = sqrtf(s2) < h.  Accepted set and
+// pre(j), x_j - x_i) when s2 <= s_max, i.e. when d = sqrtf(s2) < h.  Accepted set and
 // visiting order are those of for_neighbors, so sums are bit-identical.
 // Candidates are addressed by 32-bit byte offsets from xs (particle counts
 // stay below 2^28); pass 1 reads up to 7 entries past a row's end, so xs
@@ -200,7 +200,7 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
                 const uint32_t j = w0 + kk[t];
                 const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
                 const float s2 = dx * dx + dy * dy + dz * dz;
-                if (s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t]);
+                if (s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t], make_float3(dx, dy, dz));
             }
         }
     }
@@ -270,6 +270,34 @@ __device__ __forceinline__ uint32_t table_index(float s2, const float* __restric
     const float qa = __builtin_amdgcn_sqrtf(s2) * inv_div - 1e-3f;
     const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 1));
     return g + (s2 >= thr[g + 1u] ? 1u : 0u);
+}
+
+// The same index when d = sqrtf(s2) is already at hand (a closer first guess).
+__device__ __forceinline__ uint32_t table_index_d(float s2, float d, const float* __restrict__ thr, float inv_div) {
+    const float qa = d * inv_div - 1e-3f;
+    const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 1));
+    return g + (s2 >= thr[g + 1u] ? 1u : 0u);
+}
+
+// Correctly rounded square root (sqrtf).  For s >= 2^-96 the hardware
+// estimate r is within one ulp of sqrt(s) and needs no range scaling; of r
+// and its float neighbours lo < r < hi the nearest to sqrt(s) is lo when
+// s <= lo * r, hi when s > hi * r, else r (signs of fused residuals).  The
+// products are the squares of the geometric means of neighbouring
+// candidates; each lies ulp^2 / 4 below the square of the arithmetic
+// midpoint it stands for, and no float s falls in that gap, so the choice is
+// the rounding to nearest (checked exhaustively over [2^-96, 2^64) for every
+// estimate within one ulp by tests/test_sqrt_rn.py).
+__device__ __forceinline__ float sqrt_rn(float s) {
+    if (s >= 0x1p-96f) {
+        const float r = __builtin_amdgcn_sqrtf(s);
+        const float lo = __uint_as_float(__float_as_uint(r) - 1u);
+        const float hi = __uint_as_float(__float_as_uint(r) + 1u);
+        if (__builtin_fmaf(-lo, r, s) <= 0.f) return lo;
+        if (__builtin_fmaf(-hi, r, s) > 0.f) return hi;
+        return r;
+    }
+    return sqrtf(s);
 }
 
 // ---------------------------------------------------------- colliders --
@@ -404,17 +432,20 @@ __device__ __forceinline__ float4 predict(float4 x, float4 v, float4 vh, bool fi
 
 __device__ __forceinline__ bool step_dead(const DevGrid* g) { return g->status != 0u; }
 
-// Pressure-force direction (x_i - x_j) / d (SPHKernels.cl:406,865).  Two
+// Pressure-force direction (x_i - x_j) / d (SPHKernels.cl:406,865), from the
+// stencil's differences dl = x_j - x_i: -dl equals x_i - x_j except that an
+// exact zero difference comes out as -0, and a -0 quotient adds nothing
+// different to the force sums, which start at +0 and so never hold -0.  Two
 // distinct particles at exactly the same point (d == 0; the box collider snaps
 // every particle crossing three walls onto one corner point) would give 0/0 in
 // the reference; the pair gets the antisymmetric unit direction +y for the
 // higher sorted index and -y for the lower one, which separates it.
-__device__ __forceinline__ void pair_dir(const float4& xi, const float4& xj, float d, uint32_t i, uint32_t j,
-                                         float& ux, float& uy, float& uz) {
+__device__ __forceinline__ void pair_dir(const float3& dl, float d, uint32_t i, uint32_t j, float& ux, float& uy,
+                                         float& uz) {
     if (d > 0.f) {
-        ux = (xi.x - xj.x) / d;
-        uy = (xi.y - xj.y) / d;
-        uz = (xi.z - xj.z) / d;
+        ux = (-dl.x) / d;
+        uy = (-dl.y) / d;
+        uz = (-dl.z) / d;
     } else {
         ux = 0.f;
         uy = i > j ? 1.f : -1.f;
@@ -863,7 +894,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     float sum = 0.f;
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                          [](uint32_t) { return true; }, [](uint32_t) { return 0; },
-                         [&](uint32_t, const float4&, float s2, int) { sum += sW[table_index(s2, sT, c.inv_div)]; });
+                         [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
     const float rho = c.mass * sum;
     float p = c.stiffness * (rho - c.rest_density);
     if (p < 0.f) p *= c.npf;
@@ -909,7 +940,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     // the density pass recorded this stencil's masks at the same positions
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                          MC_REPLAY, [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
-                         [&](uint32_t j, const float4& xj, float s2, const float4& vj) {
+                         [&](uint32_t j, const float4& xj, float s2, const float4& vj, const float3&) {
         const uint32_t idx = table_index(s2, sT, c.inv_div);
         const float rj = xj.w;
         const float w1 = sW1[idx];
@@ -989,11 +1020,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                          MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
-                         [&](uint32_t j, const float4& xj, float s2, float qj) {
-        const float d = sqrtf(s2);
+                         [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
+        const float d = sqrt_rn(s2);
         float ux, uy, uz;
-        pair_dir(xi, xj, d, i, j, ux, uy, uz);
-        const float w = sP1[table_index(s2, sT, c.inv_div)];
+        pair_dir(dl, d, i, j, ux, uy, uz);
+        const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
         const float sc = tf + qj;
         pfx += ux * sc * w;
         pfy += uy * sc * w;
@@ -1060,7 +1091,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         // recorded for the pressure pass of this correction (same positions and cells)
         for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_RECORD, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
-                             [&](uint32_t, const float4&, float s2, int) { sum += sW[table_index(s2, sT, c.inv_div)]; });
+                             [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
         const float rp = c.mass * sum;
         float e = rp - c.rest_density;
         float dp = c.delta * e;
@@ -1213,11 +1244,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     // the predicted-density pass of this correction recorded the masks
     for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                          MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
-                         [&](uint32_t j, const float4& xj, float s2, float qj) {
-        const float d = sqrtf(s2);
+                         [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
+        const float d = sqrt_rn(s2);
         float ux, uy, uz;
-        pair_dir(xi, xj, d, i, j, ux, uy, uz);
-        const float w = sP1[table_index(s2, sT, c.inv_div)];
+        pair_dir(dl, d, i, j, ux, uy, uz);
+        const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
         const float sc = tf + qj;
         pfx += ux * sc * w;
         pfy += uy * sc * w;
