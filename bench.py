@@ -282,6 +282,26 @@ def copy_calibration():
 
 
 TRAFFIC = os.path.join(ROOT, "profiles", "pcisph_dam_break_524k_traffic.json")
+ISSUE = os.path.join(ROOT, "profiles", "pcisph_dam_break_524k_issue.json")
+# VALU issue peak: 256 CUs x 4 SIMDs, one wave64 instruction per SIMD every 4 cycles at 2.4 GHz
+VALU_PEAK = 256 * 4 * 2.4e9 / 4
+
+
+def issue_roofline(kernel: str):
+    """The stencil kernels are bound by instruction issue, not HBM: VALU issue
+    of `kernel` from the committed rocprofv3 counter summary (SQ_INSTS_VALU and
+    SQ_INSTS_VALU_TRANS_F32 per launch over that run's own launch time; a
+    transcendental occupies its SIMD four issue slots), or None."""
+    try:
+        with open(ISSUE) as f:
+            k = json.load(f)["kernels"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    slots = k["valu"] + 3.0 * k["valu_trans"]
+    achieved = slots / (k["avg_us"] * 1e-6)
+    return {"bound": "valu", "kernel": kernel, "achieved": achieved, "peak": VALU_PEAK, "unit": "wave-instr/s",
+            "frac": achieved / VALU_PEAK, "valu_per_launch": k["valu"], "transcendental_per_launch": k["valu_trans"],
+            "counter_run_avg_us": k["avg_us"], "source": os.path.relpath(ISSUE, ROOT)}
 
 
 def measured_traffic(kernel: str):
@@ -380,6 +400,7 @@ def main():
         "learned": learned,
         "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned is not None else None,
         "roofline": roof,
+        "issue_roofline": issue_roofline(dom) if dom else None,
         "step_roofline": {"achieved_GBps": step_roof, "frac": step_roof / HBM_PEAK_GBS,
                           "bytes_per_particle_step": step_bytes(mean_k)},
         "kernels": kernels,
