@@ -116,3 +116,22 @@ def test_dam_break_524k_30_steps_bit_exact(gpu, oracle):
     assert np.isfinite(p["x"]).all()
     assert ulp_diff(p["x"], a.x) == 0
     assert ulp_diff(p["v"], a.v) == 0
+
+
+def test_obstacles_2m_3_steps_bit_exact(gpu, oracle):
+    """The largest configured scene, 2,097,152 particles in a 2 x 1 x 1 m box
+    with three obstacle spheres and a 20 degree plane: 3 steps on both sides,
+    state and iteration counts bit-identical (the largest particle count and
+    cell table of the configurations)."""
+    sc = scenes.pcisph_obstacles_2m()
+    st = oracle.State.from_scene(sc.x)
+    a = st.copy()
+    status, its, _ = oracle.run_ocl(sc.params, sc.colliders, sc.dt, 3, a, 0)
+    assert status == 0
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 3)
+    p = s.get_particles()
+    print("iterations gpu", [x.iterations for x in stats], "orc", its.tolist(), flush=True)
+    assert [x.iterations for x in stats] == its.tolist()
+    for k, ref in (("x", a.x), ("v", a.v), ("vhalf", a.vh)):
+        assert ulp_diff(p[k], ref) == 0, k
+    np.testing.assert_array_equal(p["first"], a.first)
