@@ -128,8 +128,19 @@ int lps_index_thresholds(const lps_params* params, float* thr /* [LPS_TABLE_SIZE
 #define LPS_MAX_PARTICLES ((1u << 28) - 1u)
 int lps_set_particles(lps_ctx* ctx, const float* x, const float* v, const float* vhalf,
                       const uint8_t* first, uint32_t n);
+/* Particles added mid-run (SPHSolver::addParticle between steps, as the
+ * viewer's emitting scenarios do, e.g. WaterfallScenario.cpp:55-74): appended
+ * after the current particles, which keep their state and sorted order; the
+ * reference re-uploads all particles instead (m_hasParticleDataChanged,
+ * SPHSolver.cpp:791).  Device storage grows by half again when full.  One
+ * scene only (LPS_ERR_STATE for ensembles or an empty context); vhalf and
+ * first as in lps_set_particles; new particles get ids counting on from the
+ * last.  Moving colliders need only lps_set_colliders between steps. */
+int lps_add_particles(lps_ctx* ctx, const float* x, const float* v, const float* vhalf, const uint8_t* first,
+                      uint32_t n);
 /* Readback of SPHSolver::onEndUpdate (SPHSolver.cpp:398-428), in sorted order.
- * Any pointer may be NULL; id returns each particle's index at lps_set_particles. */
+ * Any pointer may be NULL; id returns each particle's index at lps_set_particles
+ * (particles added later count on from there). */
 int lps_get_particles(lps_ctx* ctx, float* x, float* v, float* vhalf, uint8_t* first, uint32_t* id,
                       uint32_t n);
 int lps_get_count(lps_ctx* ctx, uint32_t* n);

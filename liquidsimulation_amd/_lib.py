@@ -49,6 +49,7 @@ def lib():
     L.lps_get_delta.argtypes = [vp, C.c_float, C.POINTER(C.c_float)]
     L.lps_index_thresholds.argtypes = [C.POINTER(CParams), _f32p]
     L.lps_set_particles.argtypes = [vp, _f32p, vp, vp, vp, C.c_uint32]
+    L.lps_add_particles.argtypes = [vp, _f32p, vp, vp, vp, C.c_uint32]
     L.lps_get_particles.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32]
     L.lps_get_count.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.lps_set_colliders.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32]

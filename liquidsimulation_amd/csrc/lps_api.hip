@@ -42,6 +42,7 @@ struct lps_ctx {
 
     uint32_t n = 0;         // particle slots: all scenes, each padded to whole blocks
     uint32_t n_real = 0;    // particles
+    uint32_t next_id = 0;   // id of the next particle added (ids number particles in order of arrival)
     uint32_t cap_n = 0;     // allocated slot capacity
     uint32_t cell_cap = 0;  // allocated cell-table entries (plus one sentinel)
     uint32_t passes = 0;    // radix passes for cell_cap
@@ -951,6 +952,7 @@ int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const f
     HIPC(hipMemset(c->d_sgrid, 0, sizeof(SceneGrid) * nscenes));
     c->n = n;
     c->n_real = (uint32_t)total;
+    c->next_id = (uint32_t)total;
     c->have_step = false;
     c->epoch++;
     if ((st = upload_scenes(c))) return st;
@@ -963,6 +965,97 @@ int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* v
                       uint32_t n) {
     if (!c || !x || n == 0 || n > LPS_MAX_PARTICLES - LPS_BLOCK) return LPS_ERR_INVALID;
     return lps_set_scenes(c, 1u, &n, nullptr, x, v, vhalf, first);
+}
+
+int lps_add_particles(lps_ctx* c, const float* x, const float* v, const float* vhalf, const uint8_t* first,
+                      uint32_t n) {
+    if (!c || !x || n == 0) return LPS_ERR_INVALID;
+    if (c->h_sinfo.size() != 1u || c->n_real == 0) return LPS_ERR_STATE;
+    for (uint64_t k = 0; k < 3ull * n; k++)
+        if (!std::isfinite(x[k]) || (v && !std::isfinite(v[k])) || (vhalf && !std::isfinite(vhalf[k])))
+            return LPS_ERR_INVALID;
+    const uint32_t old = c->n_real;
+    const uint64_t total = (uint64_t)old + n;
+    const uint64_t slots = (total + LPS_BLOCK - 1u) / LPS_BLOCK * LPS_BLOCK;
+    if (slots > LPS_MAX_PARTICLES) return LPS_ERR_INVALID;
+    const hipStream_t s = c->stream;
+    HIPC(hipStreamSynchronize(s));
+    int st;
+    if (slots > c->cap_n) {
+        // grow by half again, keeping the current state (slots 0 .. old)
+        uint64_t cap = std::max<uint64_t>(slots, (uint64_t)c->cap_n + c->cap_n / 2u);
+        cap = std::min<uint64_t>((cap + LPS_BLOCK - 1u) / LPS_BLOCK * LPS_BLOCK,
+                                 (uint64_t)LPS_MAX_PARTICLES / LPS_BLOCK * LPS_BLOCK);
+        const int b = c->cur;
+        float4 *sp = nullptr, *sv = nullptr, *sh = nullptr;
+        uint8_t* sf = nullptr;
+        uint32_t* si = nullptr;
+        float* spp = nullptr;
+        auto release = [&]() { dfree(sp); dfree(sv); dfree(sh); dfree(sf); dfree(si); dfree(spp); };
+        if ((st = dalloc(&sp, old)) || (st = dalloc(&sv, old)) || (st = dalloc(&sh, old)) || (st = dalloc(&sf, old)) ||
+            (st = dalloc(&si, old)) || (st = dalloc(&spp, old))) {
+            release();
+            return st;
+        }
+        hipError_t e = hipSuccess;
+        auto cp = [&](void* dst, const void* src, size_t bytes) {
+            if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s);
+        };
+        cp(sp, c->pos[b], sizeof(float4) * old); cp(sv, c->vel[b], sizeof(float4) * old);
+        cp(sh, c->vh[b], sizeof(float4) * old); cp(sf, c->first[b], old);
+        cp(si, c->id[b], sizeof(uint32_t) * old); cp(spp, c->pprev[b], sizeof(float) * old);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess || (st = ensure_particles(c, (uint32_t)cap))) {
+            release();
+            return e != hipSuccess ? hip_status(e) : st;
+        }
+        c->cur = 0;
+        cp(c->pos[0], sp, sizeof(float4) * old); cp(c->vel[0], sv, sizeof(float4) * old);
+        cp(c->vh[0], sh, sizeof(float4) * old); cp(c->first[0], sf, old);
+        cp(c->id[0], si, sizeof(uint32_t) * old); cp(c->pprev[0], spp, sizeof(float) * old);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        release();
+        if (e != hipSuccess) return hip_status(e);
+    }
+    // the new particles take slots old .. total - 1 of the current state
+    const int b = c->cur;
+    std::vector<float4> buf(n);
+    auto up4 = [&](float4* dst, const float* src) -> int {
+        for (uint32_t i = 0; i < n; i++)
+            buf[i] = src ? make_float4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+        HIPC(hipMemcpy(dst + old, buf.data(), sizeof(float4) * n, hipMemcpyHostToDevice));
+        return LPS_OK;
+    };
+    if ((st = up4(c->pos[b], x)) || (st = up4(c->vel[b], v)) || (st = up4(c->vh[b], vhalf))) return st;
+    std::vector<uint8_t> f(n, 1);
+    if (first)
+        for (uint32_t i = 0; i < n; i++) f[i] = first[i] ? 1 : 0;
+    std::vector<uint32_t> ids(n);
+    for (uint32_t i = 0; i < n; i++) ids[i] = c->next_id + i;
+    HIPC(hipMemcpy(c->first[b] + old, f.data(), n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(c->id[b] + old, ids.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemset(c->pprev[b] + old, 0, sizeof(float) * n));
+    c->next_id += n;
+    // layout: one scene over the grown slot range
+    const uint32_t nslots = (uint32_t)slots;
+    SceneInfo& si = c->h_sinfo[0];
+    si.count = (uint32_t)total;
+    si.padded = nslots;
+    if (nslots != c->n) {
+        if ((st = dalloc(&c->d_bscene, nslots / LPS_BLOCK))) return st;
+        if ((st = dalloc(&c->d_actlist, 2 * (size_t)(nslots / LPS_BLOCK)))) return st;
+        HIPC(hipMemset(c->d_bscene, 0, nslots / LPS_BLOCK));
+    }
+    c->n = nslots;
+    c->n_real = (uint32_t)total;
+    c->have_step = false;
+    c->last_fp_valid = 0;
+    c->feat_valid = c->seed_valid = false;
+    c->epoch++;
+    if ((st = upload_scenes(c))) return st;
+    lps_dev::launch_bbox(s, c->pos[b], nslots, scenes(c), c->bbox_part);
+    HIPC(hipStreamSynchronize(s));
+    return LPS_OK;
 }
 
 int lps_get_scene_count(lps_ctx* c, uint32_t* nscenes) {

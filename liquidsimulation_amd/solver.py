@@ -144,6 +144,23 @@ class SPHSolver:
         self._n = n
         self._pending = []
 
+    def add_particles(self, x, v=None, vhalf=None, first=None):
+        """Append particles to the running state on the device (lps_add_particles);
+        the current particles keep their state and sorted order."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 3)
+        n = x.shape[0]
+        if n == 0:
+            return
+        if not self._n:
+            self.set_particles(x, v, vhalf, first)
+            return
+        va = None if v is None else np.ascontiguousarray(v, np.float32).reshape(n, 3)
+        ha = None if vhalf is None else np.ascontiguousarray(vhalf, np.float32).reshape(n, 3)
+        fa = None if first is None else np.ascontiguousarray(first, np.uint8).reshape(n)
+        self._push_colliders()
+        check(lib().lps_add_particles(self._ctx, x.reshape(-1), ptr(va), ptr(ha), ptr(fa), n), "lps_add_particles")
+        self._n += n
+
     def addParticles(self, x, v=None):
         """Queue particles (positions, velocities); uploaded at the next update."""
         x = np.asarray(x, np.float32).reshape(-1, 3)
@@ -188,20 +205,12 @@ class SPHSolver:
 
     def _flush_pending(self):
         if self._pending:
-            old = self.get_particles() if self._n else None
-            xs = [p[0] for p in self._pending]
-            vs = [p[1] for p in self._pending]
-            if old is not None:
-                xs.insert(0, old["x"])
-                vs.insert(0, old["v"])
-            x = np.concatenate(xs)
-            v = np.concatenate(vs)
-            first = np.ones(x.shape[0], np.uint8)
-            vh = np.zeros_like(x)
-            if old is not None:
-                first[: self._n] = old["first"]
-                vh[: self._n] = old["vhalf"]
-            self.set_particles(x, v, vh, first)
+            # queued particles follow the current ones, as SPHSolver::addParticle
+            # appends to the particle list
+            x = np.concatenate([p[0] for p in self._pending])
+            v = np.concatenate([p[1] for p in self._pending])
+            self._pending = []
+            self.add_particles(x, v)
         self._push_colliders()
 
     # ---- step (PhysicSolver::update)

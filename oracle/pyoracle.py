@@ -112,6 +112,8 @@ def ref():
         R.ref_delta.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float]
         R.ref_delta.restype = C.c_float
         R.ref_spawn_cube.argtypes = [_f32p, _f32p, C.c_float, _f32p, C.c_int32]
+        R.ref_spawn_sphere.argtypes = [_f32p, C.c_float, C.c_float, _f32p, C.c_int32]
+        R.ref_spawn_circle_z.argtypes = [_f32p, C.c_float, C.c_float, _f32p, C.c_int32]
         R.ref_collide.argtypes = [_f32p, C.c_int32, _f32p, C.c_int32, _i32p, _i32p, C.c_int32, C.c_float,
                                   C.c_float, C.c_float, C.c_int32, _f32p, _f32p]
         R.ref_neighbors.argtypes = [C.c_float, C.c_int32, _f32p, C.c_int32, _i32p, C.c_int32]
@@ -388,6 +390,18 @@ def ref_delta(rp, factor, rho0, dt) -> float:
 def ref_spawn_cube(pos, half, rp, cap=1 << 22):
     out = np.zeros(3 * cap, np.float32)
     n = ref().ref_spawn_cube(np.array(pos, np.float32), np.array(half, np.float32), rp, out, cap)
+    return out[: 3 * n].reshape(n, 3)
+
+
+def ref_spawn_sphere(pos, radius, rp, cap=1 << 20):
+    out = np.zeros(3 * cap, np.float32)
+    n = ref().ref_spawn_sphere(np.array(pos, np.float32), radius, rp, out, cap)
+    return out[: 3 * n].reshape(n, 3)
+
+
+def ref_spawn_circle_z(pos, radius, rp, cap=1 << 20):
+    out = np.zeros(3 * cap, np.float32)
+    n = ref().ref_spawn_circle_z(np.array(pos, np.float32), radius, rp, out, cap)
     return out[: 3 * n].reshape(n, 3)
 
 

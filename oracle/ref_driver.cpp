@@ -173,6 +173,25 @@ int ref_spawn_cube(const float* pos, const float* half, float rp, float* out, in
     return n;
 }
 
+// SPHParticleEmitter::spawnSphere / spawnCircleZ (SPHParticleEmitter.cpp:45-75, 137-162).
+static int copy_out(const std::vector<Vector3D>& v, float* out, int32_t cap) {
+    int32_t n = (int32_t)v.size();
+    for (int32_t i = 0; i < n && i < cap; i++) {
+        out[3 * i] = v[i].getX();
+        out[3 * i + 1] = v[i].getY();
+        out[3 * i + 2] = v[i].getZ();
+    }
+    return n;
+}
+
+int ref_spawn_sphere(const float* pos, float radius, float rp, float* out, int32_t cap) {
+    return copy_out(SPHParticleEmitter::spawnSphere(Vector3D(pos[0], pos[1], pos[2]), radius, rp), out, cap);
+}
+
+int ref_spawn_circle_z(const float* pos, float radius, float rp, float* out, int32_t cap) {
+    return copy_out(SPHParticleEmitter::spawnCircleZ(Vector3D(pos[0], pos[1], pos[2]), radius, rp), out, cap);
+}
+
 // Colliders applied in insertion order with the reference's collision classes.
 int ref_collide(const float* boxes, int32_t nbox, const float* spheres, int32_t nsphere, const int32_t* kinds,
                 const int32_t* index, int32_t ncoll, float r, float e, float mu, int32_t n, float* x, float* vel) {

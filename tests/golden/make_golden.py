@@ -43,6 +43,14 @@ def main():
                                          ((0.3, 0.5, 0.2), (0.05, 0.05, 0.05), 0.0125)]):
         g[f"cube{k}_in"] = np.array([*pos, *half, rp], np.float32)
         g[f"cube{k}_out"] = O.ref_spawn_cube(pos, half, rp)
+    # spawnSphere / spawnCircleZ (the scenarios' emitters; circle with the reference's offset bug)
+    for k, (pos, rad, rp) in enumerate([((0.6, 0.0, 0.0), 0.6, 0.017), ((0.25, 0.5, -0.3), 0.1, 0.017),
+                                        ((0.0, 0.0, 0.0), 0.05, 0.0125)]):
+        g[f"sphere{k}_in"] = np.array([*pos, rad, rp], np.float32)
+        g[f"sphere{k}_out"] = O.ref_spawn_sphere(pos, rad, rp)
+    for k, (pos, rad, rp) in enumerate([((-0.8, 0.3, -0.583), 0.1, 0.017), ((0.0, 0.0, 0.0), 0.05, 0.0125)]):
+        g[f"circlez{k}_in"] = np.array([*pos, rad, rp], np.float32)
+        g[f"circlez{k}_out"] = O.ref_spawn_circle_z(pos, rad, rp)
     # colliders: random particles against boxes and spheres, reference classes
     rng = np.random.default_rng(7)
     x = rng.uniform(-0.3, 1.3, size=(400, 3)).astype(np.float32)
