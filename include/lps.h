@@ -144,6 +144,16 @@ int lps_add_particles(lps_ctx* ctx, const float* x, const float* v, const float*
 int lps_get_particles(lps_ctx* ctx, float* x, float* v, float* vhalf, uint8_t* first, uint32_t* id,
                       uint32_t n);
 int lps_get_count(lps_ctx* ctx, uint32_t* n);
+/* Render interop (no counterpart in the reference, whose viewer reads every
+ * particle back to the host each frame, SPHSolver.cpp:398-428): writes the
+ * positions in lps_get_particles order into device memory, particle i's x, y,
+ * z at dst[i * stride + 0..2] (stride >= 3 floats; 4 for a vec4 instance
+ * buffer).  dst is device memory of capacity_bytes, e.g. a vertex buffer
+ * mapped with hipGraphicsGLRegisterBuffer / hipGraphicsResourceGetMappedPointer;
+ * a host pointer or a buffer too small for n particles is rejected
+ * (LPS_ERR_INVALID).  n must equal the particle count.  Returns when the copy
+ * is complete. */
+int lps_copy_positions_device(lps_ctx* ctx, float* dst, uint32_t stride, uint32_t n, uint64_t capacity_bytes);
 /* Collider upload of SPHSolver::initParallelBuffers (SPHSolver.cpp:932-1003). */
 int lps_set_colliders(lps_ctx* ctx, const lps_box* boxes, uint32_t nbox, const lps_sphere* spheres,
                       uint32_t nsphere, const lps_plane* planes, uint32_t nplane);

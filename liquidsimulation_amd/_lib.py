@@ -39,6 +39,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise LibraryMissing(f"{LIB_PATH} is not built; run `python -m liquidsimulation_amd.build`")
+    # PyTorch brings its own ROCm runtime libraries.  When both are used in one
+    # process, liblps must bind to the HIP runtime torch loads: loaded first,
+    # liblps would pull in the system's runtime, torch's libraries would then
+    # run against it and find no device.  So torch, when installed, loads first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.lps_create.argtypes = [C.POINTER(CParams), C.POINTER(vp)]
@@ -50,6 +58,7 @@ def lib():
     L.lps_index_thresholds.argtypes = [C.POINTER(CParams), _f32p]
     L.lps_set_particles.argtypes = [vp, _f32p, vp, vp, vp, C.c_uint32]
     L.lps_add_particles.argtypes = [vp, _f32p, vp, vp, vp, C.c_uint32]
+    L.lps_copy_positions_device.argtypes = [vp, vp, C.c_uint32, C.c_uint32, C.c_uint64]
     L.lps_get_particles.argtypes = [vp, vp, vp, vp, vp, vp, C.c_uint32]
     L.lps_get_count.argtypes = [vp, C.POINTER(C.c_uint32)]
     L.lps_set_colliders.argtypes = [vp, vp, C.c_uint32, vp, C.c_uint32, vp, C.c_uint32]

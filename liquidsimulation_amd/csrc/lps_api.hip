@@ -672,8 +672,11 @@ int upload_scenes(lps_ctx* c) {
     std::vector<lps_box> boxes;
     std::vector<lps_sphere> spheres;
     std::vector<lps_plane> planes;
+    uint32_t real0 = 0;
     for (size_t k = 0; k < S; k++) {
         SceneInfo& si = c->h_sinfo[k];
+        si.real0 = real0;
+        real0 += si.count;
         si.box0 = (uint32_t)boxes.size();
         si.nbox = (uint32_t)c->s_boxes[k].size();
         si.sphere0 = (uint32_t)spheres.size();
@@ -1055,6 +1058,29 @@ int lps_add_particles(lps_ctx* c, const float* x, const float* v, const float* v
     if ((st = upload_scenes(c))) return st;
     lps_dev::launch_bbox(s, c->pos[b], nslots, scenes(c), c->bbox_part);
     HIPC(hipStreamSynchronize(s));
+    return LPS_OK;
+}
+
+int lps_copy_positions_device(lps_ctx* c, float* dst, uint32_t stride, uint32_t n, uint64_t capacity_bytes) {
+    if (!c || !dst || stride < 3u || n != c->n_real || n == 0) return LPS_ERR_INVALID;
+    const uint64_t need = ((uint64_t)(n - 1u) * stride + 3u) * sizeof(float);
+    if (capacity_bytes < need) return LPS_ERR_INVALID;
+    // the destination must be device memory of at least `need` bytes
+    hipPointerAttribute_t attr{};
+    if (hipPointerGetAttributes(&attr, dst) != hipSuccess || attr.type != hipMemoryTypeDevice) {
+        (void)hipGetLastError();
+        return LPS_ERR_INVALID;
+    }
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)dst) == hipSuccess) {
+        if ((const char*)dst + need > (const char*)base + size) return LPS_ERR_INVALID;
+    } else {
+        (void)hipGetLastError();
+    }
+    lps_dev::launch_export_positions(c->stream, c->pos[c->cur], c->n, scenes(c), dst, stride);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
     return LPS_OK;
 }
 

@@ -34,6 +34,7 @@ struct SceneInfo {
     uint32_t block0;       // slot0 / LPS_BLOCK
     float gravity[3];
     uint32_t box0, nbox, sphere0, nsphere, plane0, nplane;  // ranges of the collider arrays
+    uint32_t real0;        // real particles of the scenes before this one (readback position)
 };
 
 // Per-scene grid and PCISPH loop state of the current step, written on the

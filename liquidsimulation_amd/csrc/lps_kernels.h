@@ -127,5 +127,8 @@ void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, 
                          const float* q, float4* fp, float4* xpred_next, uint32_t k);
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
                       uint32_t* bbox_part);
+// positions of the real particles, scenes back to back, to dst[r * stride + 0..2]
+void launch_export_positions(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, float* dst,
+                             uint32_t stride);
 
 }  // namespace lps_dev

@@ -1316,6 +1316,25 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     bbox_partial(bbox_part, lb, valid, p.x, p.y, p.z);
 }
 
+// Render interop: the real particles' positions, in readback order (scenes
+// back to back, each in its sorted order), into a caller's device buffer
+// (a mapped vertex buffer, a tensor) at a stride of `stride` floats; the
+// readback of SPHSolver::onEndUpdate (SPHSolver.cpp:398-428) without a host copy.
+__global__ __launch_bounds__(LPS_BLOCK) void k_export_positions(const float4* __restrict__ pos, lps_dev::Scenes S,
+                                                                float* __restrict__ dst, uint32_t stride,
+                                                                uint32_t nlogical) {
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    const BlockScene B = block_scene(S, lb);
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    if (i >= B.info.slot0 + B.info.count) return;
+    const float4 p = pos[i];
+    float* o = dst + (size_t)(B.info.real0 + (i - B.info.slot0)) * stride;
+    o[0] = p.x;
+    o[1] = p.y;
+    o[2] = p.z;
+}
+
 // ====================================================== host launchers ====
 namespace lps_dev {
 
@@ -1406,6 +1425,12 @@ void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float*
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_integrate<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.vel, a.vh, a.first, a.force, fp, a.posrho, pprev, press,
                                                    a.g, a.c, a.boxes, a.spheres, a.planes, bbox_part, a.sc, nl);
+}
+
+void launch_export_positions(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, float* dst,
+                             uint32_t stride) {
+    const uint32_t nl = cdiv(n, LPS_BLOCK);
+    k_export_positions<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, sc, dst, stride, nl);
 }
 
 }  // namespace lps_dev

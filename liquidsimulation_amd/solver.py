@@ -161,6 +161,19 @@ class SPHSolver:
         check(lib().lps_add_particles(self._ctx, x.reshape(-1), ptr(va), ptr(ha), ptr(fa), n), "lps_add_particles")
         self._n += n
 
+    def positions_into(self, dst) -> None:
+        """Writes the particle positions (get_particles order) into a CUDA
+        tensor of shape [n, k >= 3], float32, contiguous rows; no host copy
+        (lps_copy_positions_device)."""
+        import torch
+        self._flush_pending()
+        if not (isinstance(dst, torch.Tensor) and dst.is_cuda and dst.dtype == torch.float32 and dst.dim() == 2
+                and dst.shape[0] == self._n and dst.shape[1] >= 3 and dst.stride(1) == 1):
+            raise ValueError("positions_into needs a CUDA float32 tensor [n, k >= 3] with contiguous rows")
+        check(lib().lps_copy_positions_device(self._ctx, dst.data_ptr(), dst.stride(0), self._n,
+                                              dst.untyped_storage().nbytes() - dst.storage_offset() * 4),
+              "lps_copy_positions_device")
+
     def addParticles(self, x, v=None):
         """Queue particles (positions, velocities); uploaded at the next update."""
         x = np.asarray(x, np.float32).reshape(-1, 3)

@@ -104,3 +104,46 @@ def test_scenario_sph_matches_oracle(gpu, oracle, name, kw):
 @pytest.mark.parametrize("name", ["waterfall", "wave_breaker"])
 def test_scenario_pcisph_matches_oracle(gpu, oracle, name):
     _run_both(oracle, name, 16, METHOD_PCISPH)
+
+
+def test_positions_into_device_buffer_match_readback(gpu):
+    """Render interop: positions written into a device buffer (a torch tensor
+    standing in for a mapped vertex buffer) equal the host readback, for one
+    scene and for an ensemble (scenes back to back); host pointers and short
+    buffers are rejected before anything runs on the device."""
+    import torch
+
+    from liquidsimulation_amd._lib import lib
+    from liquidsimulation_amd.solver import Ensemble
+
+    sc = scenes.small_block(9, 7, 5, method=METHOD_PCISPH, dt=0.002)
+    s = solver_for(sc.params)
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x)
+    s.update(sc.dt, 3)
+    s.add_particles(scenes.block_lattice(3, 3, 3, 0.02, origin=(0.05, 0.4, 0.05)))
+    s.update(sc.dt, 2)
+    ref = s.get_particles()["x"]
+    for k in (3, 4, 7):
+        t = torch.full((ref.shape[0], k), -1.0, device="cuda", dtype=torch.float32)
+        s.positions_into(t)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(t[:, :3].cpu().numpy(), ref)
+        assert bool((t[:, 3:] == -1.0).all())
+    # rejected: a host pointer, a buffer too small, a stride below 3
+    host = np.zeros((ref.shape[0], 4), np.float32)
+    assert lib().lps_copy_positions_device(s._ctx, host.ctypes.data, 4, ref.shape[0], host.nbytes) != 0
+    small = torch.zeros((ref.shape[0] - 1, 4), device="cuda", dtype=torch.float32)
+    assert lib().lps_copy_positions_device(s._ctx, small.data_ptr(), 4, ref.shape[0], small.numel() * 4) != 0
+    assert lib().lps_copy_positions_device(s._ctx, small.data_ptr(), 2, ref.shape[0], 1 << 30) != 0
+    s.close()
+    a, b = scenes.small_block(6, 6, 6, seed=1), scenes.small_block(5, 9, 4, seed=2)
+    e = Ensemble(a.params)
+    e.set_scenes([a.x, b.x + np.float32(0.0)])
+    e.update(a.dt, 2)
+    ref = e.get_particles()["x"]
+    t = torch.empty((ref.shape[0], 4), device="cuda", dtype=torch.float32)
+    e.positions_into(t)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(t[:, :3].cpu().numpy(), ref)
+    e.close()
