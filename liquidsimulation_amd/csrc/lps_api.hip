@@ -94,6 +94,7 @@ struct lps_ctx {
     bool mlp_loaded = false;
     bool capture = false;         // compute features every PCISPH step (for lps_dump_pairs)
     float* d_mlp = nullptr;       // params [LPS_MLP_PARAMS] then fscale [2 * LPS_MLP_IN]
+    uint8_t* d_mlp_packed = nullptr;  // BF16 inference copy of d_mlp (lps_dev::mlp_packed_bytes())
     float* feat = nullptr;        // [cap_n][LPS_MLP_IN] features of the current step
     float* seed = nullptr;        // [cap_n] seeded pressure of the current step
     bool feat_valid = false;      // the last step wrote features (and final pressure)
@@ -345,7 +346,7 @@ void free_all(lps_ctx* c) {
     dfree(c->mk_mask); dfree(c->mk_ok);
     dfree(c->run_end); dfree(c->cell_lb); dfree(c->block_max);
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
-    dfree(c->d_mlp); dfree(c->feat); dfree(c->seed); dfree(c->coll_x); dfree(c->coll_t);
+    dfree(c->d_mlp); dfree(c->d_mlp_packed); dfree(c->feat); dfree(c->seed); dfree(c->coll_x); dfree(c->coll_t);
     dfree(c->d_grad); dfree(c->d_m1); dfree(c->d_m2); dfree(c->d_loss); dfree(c->d_stats);
     dfree(c->d_sinfo); dfree(c->d_sgrid); dfree(c->d_bscene); dfree(c->d_actlist);
     if (c->h_sgrid) (void)hipHostFree(c->h_sgrid);
@@ -521,7 +522,7 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     if (kk == 1u && seeded) {
         // the first correction's pressure update is replaced by the seed
         Timed t(c, "mlp_seed");
-        lps_dev::launch_mlp_seed(s, mlp_args(c), c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
+        lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
                                  delta * c->params.rest_density,
                                  c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c));
     }
@@ -1450,12 +1451,15 @@ int lps_mlp_set_weights(lps_ctx* c, const float* params, const float* feature_sc
     for (uint32_t k = 0; k < 2 * LPS_MLP_IN; k++)
         if (!std::isfinite(feature_scale[k])) return LPS_ERR_INVALID;
     if (!c->d_mlp) HIPC(hipMalloc((void**)&c->d_mlp, sizeof(float) * (LPS_MLP_PARAMS + 2 * LPS_MLP_IN)));
+    if (!c->d_mlp_packed) HIPC(hipMalloc((void**)&c->d_mlp_packed, lps_dev::mlp_packed_bytes()));
     c->mlp_w.assign(params, params + LPS_MLP_PARAMS);
     c->mlp_fscale.assign(feature_scale, feature_scale + 2 * LPS_MLP_IN);
     c->mlp_tscale = target_scale;
     HIPC(hipMemcpyAsync(c->d_mlp, c->mlp_w.data(), sizeof(float) * LPS_MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
     HIPC(hipMemcpyAsync(c->d_mlp + LPS_MLP_PARAMS, c->mlp_fscale.data(), sizeof(float) * 2 * LPS_MLP_IN,
                         hipMemcpyHostToDevice, c->stream));
+    lps_dev::launch_mlp_pack(c->stream, mlp_args(c), c->d_mlp_packed);
+    HIPC(hipGetLastError());
     HIPC(hipStreamSynchronize(c->stream));
     c->mlp_loaded = true;
     c->epoch++;
@@ -1535,6 +1539,7 @@ namespace {
 int ensure_train_buffers(lps_ctx* c) {
     int st;
     if (!c->d_mlp && (st = dalloc(&c->d_mlp, LPS_MLP_PARAMS + 2 * LPS_MLP_IN))) return st;
+    if (!c->d_mlp_packed && (st = dalloc(&c->d_mlp_packed, lps_dev::mlp_packed_bytes()))) return st;
     if (!c->d_grad && (st = dalloc(&c->d_grad, LPS_MLP_PARAMS))) return st;
     if (!c->d_m1 && (st = dalloc(&c->d_m1, LPS_MLP_PARAMS))) return st;
     if (!c->d_m2 && (st = dalloc(&c->d_m2, LPS_MLP_PARAMS))) return st;
@@ -1634,6 +1639,7 @@ int lps_mlp_train(lps_ctx* c, const lps_mlp_train_cfg* cfg, float* epoch_loss) {
         HIPC(hipStreamSynchronize(s));
         if (epoch_loss) epoch_loss[e] = loss / (float)per_epoch;
     }
+    lps_dev::launch_mlp_pack(s, m, c->d_mlp_packed);
     HIPC(hipGetLastError());
     HIPC(hipMemcpyAsync(c->mlp_w.data(), c->d_mlp, sizeof(float) * LPS_MLP_PARAMS, hipMemcpyDeviceToHost, s));
     HIPC(hipStreamSynchronize(s));
@@ -1659,7 +1665,7 @@ int lps_mlp_forward(lps_ctx* c, const float* x, uint32_t n, float* y) {
     if (!st && hipMemcpy(dx, x, sizeof(float) * LPS_MLP_IN * (size_t)n, hipMemcpyHostToDevice) != hipSuccess)
         st = LPS_ERR_HIP;
     if (!st) {
-        lps_dev::launch_mlp_forward(c->stream, mlp_args(c), dx, n, dy);
+        lps_dev::launch_mlp_forward(c->stream, c->d_mlp_packed, dx, n, dy);
         if (hipMemcpyAsync(y, dy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
             st = LPS_ERR_HIP;

@@ -111,11 +111,14 @@ struct RowMap {
 // p_i = pscale * tscale * MLP(features_i) (non-negative when negative
 // pressures are disabled, 0 when not finite), q_i = p_i / rho*_i^2,
 // seed_i = p_i.  No-op once the step has stopped or failed.
-void launch_mlp_seed(hipStream_t s, const MlpArgs& m, const float* feat, uint32_t n, float* press, float* q,
-                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
+void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t n, float* press,
+                     float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
                      const Scenes& sc);
+// BF16 inference weights (k_mlp_seed's LDS layout) from the FP32 master copy
+size_t mlp_packed_bytes();
+void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed);
 // batch inference (raw output, before tscale) / gradients, for training and tests
-void launch_mlp_forward(hipStream_t s, const MlpArgs& m, const float* x, uint32_t n, float* y);
+void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y);
 void launch_mlp_grad(hipStream_t s, const MlpArgs& m, const float* x, const float* t, const RowMap& rm, uint32_t n,
                      float over_weight, float* grad, float* loss);
 void launch_adam(hipStream_t s, float* params, float* grad, float* m1, float* m2, uint32_t count, float lr,

@@ -131,6 +131,90 @@ __device__ __forceinline__ void forward(const MlpSmem& S, const float* __restric
     F.y = part + S.b3;
 }
 
+// The inference weights in their BF16 LDS layout, packed once per change of
+// the FP32 master copy (k_mlp_pack) so that the seed kernel copies them
+// instead of converting them in every block.
+struct alignas(16) MlpPacked {
+    unsigned short w1[HID * IN];    // [neuron][feature]
+    unsigned short w2[HID * HID];   // [out][in]
+    float b1[HID], b2[HID], w3[HID];
+    float mean[IN], istd[IN];
+    float b3;
+    float pad_[3];
+};
+static_assert(sizeof(MlpPacked) % 16 == 0, "packed weights are copied as uint4");
+
+// Inference of one group of 16 samples (column l%16), the output only: the
+// arithmetic of forward() (same products, order and roundings) without
+// keeping the pre-activations and hidden tiles the gradient needs.
+__device__ __forceinline__ float infer(const MlpPacked& S, const float* __restrict__ xrow, bool valid) {
+    const uint32_t l = threadIdx.x & 63u, g = l >> 4, r16 = l & 15u;
+    bf16x4 x;
+    for (int r = 0; r < 4; r++) {
+        const uint32_t f = 4u * g + (uint32_t)r;
+        const float v = valid ? (xrow[f] - S.mean[f]) * S.istd[f] : 0.f;
+        x[r] = (short)f2bf(v);
+    }
+    bf16x4 h1[4];
+    for (int mt = 0; mt < 4; mt++) {
+        const f32x4 y = mfma(load4(&S.w1[(mt * 16 + r16) * IN + 4 * g]), x, f32x4{0.f, 0.f, 0.f, 0.f});
+        for (int r = 0; r < 4; r++) h1[mt][r] = (short)f2bf(fmaxf(y[r] + S.b1[mt * 16 + 4 * g + r], 0.f));
+    }
+    float part = 0.f;
+    for (int mt = 0; mt < 4; mt++) {
+        f32x4 acc{0.f, 0.f, 0.f, 0.f};
+        for (int kt = 0; kt < 4; kt++) acc = mfma(load4(&S.w2[(mt * 16 + r16) * HID + kt * 16 + 4 * g]), h1[kt], acc);
+        for (int r = 0; r < 4; r++) {
+            const float h2 = bf2f(f2bf(fmaxf(acc[r] + S.b2[mt * 16 + 4 * g + r], 0.f)));
+            part += S.w3[mt * 16 + 4 * g + r] * h2;
+        }
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    return part + S.b3;
+}
+
+__global__ __launch_bounds__(LPS_BLOCK) void k_mlp_pack(const float* __restrict__ params,
+                                                        const float* __restrict__ fscale, MlpPacked* __restrict__ out) {
+    for (uint32_t k = threadIdx.x; k < HID * IN; k += blockDim.x) out->w1[k] = f2bf(params[OFF_W1 + k]);
+    for (uint32_t k = threadIdx.x; k < HID * HID; k += blockDim.x) out->w2[k] = f2bf(params[OFF_W2 + k]);
+    for (uint32_t k = threadIdx.x; k < HID; k += blockDim.x) {
+        out->b1[k] = params[OFF_B1 + k];
+        out->b2[k] = params[OFF_B2 + k];
+        out->w3[k] = rbf(params[OFF_W3 + k]);
+    }
+    for (uint32_t k = threadIdx.x; k < IN; k += blockDim.x) {
+        out->mean[k] = fscale[k];
+        out->istd[k] = fscale[IN + k];
+    }
+    if (threadIdx.x == 0) {
+        out->b3 = params[OFF_B3];
+        out->pad_[0] = out->pad_[1] = out->pad_[2] = 0.f;
+    }
+}
+
+__device__ __forceinline__ void load_packed(MlpPacked& S, const MlpPacked* __restrict__ packed) {
+    const uint4* src = reinterpret_cast<const uint4*>(packed);
+    uint4* dst = reinterpret_cast<uint4*>(&S);
+    for (uint32_t k = threadIdx.x; k < sizeof(MlpPacked) / 16u; k += blockDim.x) dst[k] = src[k];
+}
+
+// Network output (in units of the target scale) of n rows of x.
+__global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(const MlpPacked* __restrict__ packed,
+                                                           const float* __restrict__ x, uint32_t n,
+                                                           float* __restrict__ y) {
+    __shared__ MlpPacked S;
+    load_packed(S, packed);
+    __syncthreads();
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
+    for (uint32_t bb = blockIdx.x * LPS_BLOCK; bb < n; bb += gridDim.x * LPS_BLOCK)
+        for (uint32_t grp = 0; grp < 4; grp++) {
+            const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
+            const float v = infer(S, x + (size_t)min(p, n - 1u) * IN, p < n);
+            if (p < n && (l >> 4) == 0u) y[p] = v;
+        }
+}
+
 }  // namespace
 
 // Seed of the PCISPH pressure: p_i = pscale * tscale * MLP(features_i) with
@@ -138,14 +222,15 @@ __device__ __forceinline__ void forward(const MlpSmem& S, const float* __restric
 // makes it independent of particle size and time step), clamped at zero when
 // negative pressures are disabled, non-finite values replaced by 0;
 // q_i = p_i / rho*_i^2 for the pressure-force pass.
-__global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(MlpArgs m, const float* __restrict__ feat, uint32_t n,
+__global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restrict__ packed, float tscale,
+                                                        const float* __restrict__ feat, uint32_t n,
                                                         float* __restrict__ press, float* __restrict__ q,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
                                                         float pscale, int clamp_nonneg, const DevGrid* g,
                                                         lps_dev::Scenes sc) {
     if (g->status != 0u || g->running == 0u) return;
-    __shared__ MlpSmem S;
-    load_weights(S, m, false);
+    __shared__ MlpPacked S;
+    load_packed(S, packed);
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
     // one 256-slot chunk is one particle block, so it belongs to one scene;
@@ -156,10 +241,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(MlpArgs m, const float* 
         const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
-            Fwd F;
-            forward(S, feat + (size_t)min(p, n - 1u) * IN, p < end, F);
+            const float y = infer(S, feat + (size_t)min(p, n - 1u) * IN, p < end);
             if (p < end && (l >> 4) == 0u) {
-                float v = F.y * m.tscale * pscale;
+                float v = y * tscale * pscale;
                 if (!isfinite(v)) v = 0.f;
                 if (clamp_nonneg) v = fmaxf(v, 0.f);
                 const float rp = rho_pred[p];
@@ -171,20 +255,6 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(MlpArgs m, const float* 
     }
 }
 
-__global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(MlpArgs m, const float* __restrict__ x, uint32_t n,
-                                                           float* __restrict__ y) {
-    __shared__ MlpSmem S;
-    load_weights(S, m, false);
-    __syncthreads();
-    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    for (uint32_t bb = blockIdx.x * LPS_BLOCK; bb < n; bb += gridDim.x * LPS_BLOCK)
-        for (uint32_t grp = 0; grp < 4; grp++) {
-            const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
-            Fwd F;
-            forward(S, x + (size_t)min(p, n - 1u) * IN, p < n, F);
-            if (p < n && (l >> 4) == 0u) y[p] = F.y;
-        }
-}
 
 // Gradient of the weighted mean squared error mean(w d^2), d = y - t/tscale,
 // w = over_weight where d > 0 and 1 elsewhere, over a batch of n rows of x / t
@@ -378,14 +448,21 @@ uint32_t mlp_grid(uint64_t n) {
 }
 }  // namespace
 
-void launch_mlp_seed(hipStream_t s, const MlpArgs& m, const float* feat, uint32_t n, float* press, float* q,
-                     float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
-                     const Scenes& sc) {
-    k_mlp_seed<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(m, feat, n, press, q, seed, rho_pred, pscale, clamp_nonneg, g, sc);
+size_t mlp_packed_bytes() { return sizeof(MlpPacked); }
+
+void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed) {
+    k_mlp_pack<<<1, LPS_BLOCK, 0, s>>>(m.params, m.fscale, static_cast<MlpPacked*>(packed));
 }
 
-void launch_mlp_forward(hipStream_t s, const MlpArgs& m, const float* x, uint32_t n, float* y) {
-    k_mlp_forward<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(m, x, n, y);
+void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t n, float* press,
+                     float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
+                     const Scenes& sc) {
+    k_mlp_seed<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale, feat, n, press, q,
+                                                 seed, rho_pred, pscale, clamp_nonneg, g, sc);
+}
+
+void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y) {
+    k_mlp_forward<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), x, n, y);
 }
 
 void launch_mlp_grad(hipStream_t s, const MlpArgs& m, const float* x, const float* t, const RowMap& rm, uint32_t n,
