@@ -17,6 +17,7 @@
 #include <climits>
 
 #include "lps_common.h"
+#include "lps_div.h"
 #include "lps_kernels.h"
 
 #define LPS_XCDS 8u
@@ -434,18 +435,31 @@ __device__ __forceinline__ bool step_dead(const DevGrid* g) { return g->status !
 
 // Pressure-force direction (x_i - x_j) / d (SPHKernels.cl:406,865), from the
 // stencil's differences dl = x_j - x_i: -dl equals x_i - x_j except that an
-// exact zero difference comes out as -0, and a -0 quotient adds nothing
-// different to the force sums, which start at +0 and so never hold -0.  Two
-// distinct particles at exactly the same point (d == 0; the box collider snaps
-// every particle crossing three walls onto one corner point) would give 0/0 in
-// the reference; the pair gets the antisymmetric unit direction +y for the
-// higher sorted index and -y for the lower one, which separates it.
-__device__ __forceinline__ void pair_dir(const float3& dl, float d, uint32_t i, uint32_t j, float& ux, float& uy,
-                                         float& uz) {
+// exact zero difference comes out as -0.  The three quotients share one
+// reciprocal of d (lps_div.h) where that is exact, which also turns a zero
+// numerator into +0 as in the reference; the IEEE division elsewhere gives
+// -0 for it, and a -0 quotient adds nothing different to the force sums,
+// which start at +0 and so never hold -0.  `fastdiv` is the caller's uniform
+// bound d < 2^40.  Two distinct particles at exactly the same point (d == 0;
+// the box collider snaps every particle crossing three walls onto one corner
+// point) would give 0/0 in the reference; the pair gets the antisymmetric
+// unit direction +y for the higher sorted index and -y for the lower one,
+// which separates it.
+__device__ __forceinline__ void pair_dir(const float3& dl, float d, uint32_t i, uint32_t j, bool fastdiv, float& ux,
+                                         float& uy, float& uz) {
     if (d > 0.f) {
-        ux = (-dl.x) / d;
-        uy = (-dl.y) / d;
-        uz = (-dl.z) / d;
+        const float ax = -dl.x, ay = -dl.y, az = -dl.z;
+        const uint32_t key = min(min(lps_div::num_key(ax), lps_div::num_key(ay)), lps_div::num_key(az));
+        if (fastdiv && lps_div::div_rn_ok(d, key)) {
+            const float y = lps_div::rcp_rn(d);
+            ux = lps_div::div_rn(ax, d, y);
+            uy = lps_div::div_rn(ay, d, y);
+            uz = lps_div::div_rn(az, d, y);
+        } else {
+            ux = ax / d;
+            uy = ay / d;
+            uz = az / d;
+        }
     } else {
         ux = 0.f;
         uy = i > j ? 1.f : -1.f;
@@ -1017,13 +1031,14 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const GridView G = grid_view(g, S.grid + B.s);
     const float4 xi = posrho[i];
     const float tf = q[i];
+    const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                          MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
                          [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
         const float d = sqrt_rn(s2);
         float ux, uy, uz;
-        pair_dir(dl, d, i, j, ux, uy, uz);
+        pair_dir(dl, d, i, j, fastdiv, ux, uy, uz);
         const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
         const float sc = tf + qj;
         pfx += ux * sc * w;
@@ -1240,6 +1255,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4 xa = pos[i];
     const float4 xi = xpred[i];
     const float tf = q[i];
+    const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     // the predicted-density pass of this correction recorded the masks
     for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
@@ -1247,7 +1263,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
                          [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
         const float d = sqrt_rn(s2);
         float ux, uy, uz;
-        pair_dir(dl, d, i, j, ux, uy, uz);
+        pair_dir(dl, d, i, j, fastdiv, ux, uy, uz);
         const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
         const float sc = tf + qj;
         pfx += ux * sc * w;
