@@ -957,20 +957,33 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
                          [&](uint32_t j, const float4& xj, float s2, const float4& vj, const float3&) {
         const uint32_t idx = table_index(s2, sT, c.inv_div);
         const float rj = xj.w;
-        const float w1 = sW1[idx];
-        snx += (xi.x - xj.x) * w1 / rj;
-        sny += (xi.y - xj.y) * w1 / rj;
-        snz += (xi.z - xj.z) * w1 / rj;
-        lc += sW2[idx] / rj;
+        const float w1 = sW1[idx], wv = sV2[idx];
+        // the numerators of every quotient by rho_j (the self pair's velocity
+        // terms are zero and unused)
+        float n[8] = {(xi.x - xj.x) * w1, (xi.y - xj.y) * w1, (xi.z - xj.z) * w1, sW2[idx],
+                      (vj.x - vi.x) * wv, (vj.y - vi.y) * wv, (vj.z - vi.z) * wv, 0.f};
+        constexpr int NQ = FEAT ? 8 : 7;
+        if constexpr (FEAT)
+            n[7] = ((vj.x - vi.x) * (xi.x - xj.x) + (vj.y - vi.y) * (xi.y - xj.y) + (vj.z - vi.z) * (xi.z - xj.z)) * w1;
+        uint32_t key = lps_div::num_key(n[0]);
+        for (int t = 1; t < NQ; t++) key = min(key, lps_div::num_key(n[t]));
+        if (rj < 0x1p40f && lps_div::div_rn_ok(rj, key)) {
+            const float y = lps_div::rcp_rn(rj);
+            for (int t = 0; t < NQ; t++) n[t] = lps_div::div_rn(n[t], rj, y);
+        } else {
+            for (int t = 0; t < NQ; t++) n[t] = n[t] / rj;
+        }
+        snx += n[0];
+        sny += n[1];
+        snz += n[2];
+        lc += n[3];
         if (i != j) {
-            const float wv = sV2[idx];
-            vfx += (vj.x - vi.x) * wv / rj;
-            vfy += (vj.y - vi.y) * wv / rj;
-            vfz += (vj.z - vi.z) * wv / rj;
+            vfx += n[4];
+            vfy += n[5];
+            vfz += n[6];
             if constexpr (FEAT) {
                 nb += 1.f;
-                dv += ((vj.x - vi.x) * (xi.x - xj.x) + (vj.y - vi.y) * (xi.y - xj.y) + (vj.z - vi.z) * (xi.z - xj.z)) *
-                      w1 / rj;
+                dv += n[7];
             }
         }
     });
