@@ -346,21 +346,21 @@ def main():
         bench_ensemble(args)
         return
     scene = scenes.WORKLOADS[args.workload]()
-    learned = None
+    learned = res_l = None
     if args.mode == "both" and scene.params.method == 1 and os.path.exists(warm_start.WEIGHTS):
         w = warm_start.load()
         sl = scene.with_params(warm_start=WARM_LEARNED)
-        learned = mode_summary(run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w), sl,
-                               args.steps)
+        res_l = run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w)
+        learned = mode_summary(res_l, sl, args.steps)
     scene = scene.with_params(warm_start=WARM_NONE)
     res = run_gpu(scene, args.steps, args.warmup, not args.no_profile)
     plain = mode_summary(res, scene, args.steps)
     n = scene.n
     head = learned if learned is not None else plain
     value = head["value"]
-    its = res["its"]
     tol = scene.params.max_density_error_ratio
-    kernels = res["kernels"]
+    # the roofline is that of the run `value` comes from
+    kernels = (res_l if learned is not None else res)["kernels"]
     dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])[0] if kernels else None
     roof = None
     if dom:

@@ -63,6 +63,7 @@ struct lps_ctx {
     float4 *posrho = nullptr, *force = nullptr, *fp = nullptr, *xpred[2] = {};
     float *press = nullptr, *q = nullptr, *rho_pred = nullptr;
     uint32_t *mk_mask = nullptr, *mk_ok = nullptr;  // recorded candidate masks [LPS_MC_ROWS][cap_n], flags
+    uint2* mk_cq = nullptr;                         // compact predictions [cap_n + LPS_STENCIL_PAD]
     uint32_t *run_end = nullptr, *cell_lb = nullptr, *block_max = nullptr;
     float* tables = nullptr;
     lps_box* boxes = nullptr;
@@ -326,6 +327,7 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if ((st = dalloc(&c->rho_pred, n))) return st;
     if ((st = dalloc(&c->mk_mask, (size_t)LPS_MC_ROWS * n))) return st;
     if ((st = dalloc(&c->mk_ok, n))) return st;
+    if ((st = dalloc(&c->mk_cq, (size_t)n + LPS_STENCIL_PAD))) return st;
     // every replaying pass follows its recording pass; until then no slot
     // counts as recorded
     HIPC(hipMemsetAsync(c->mk_ok, 0, sizeof(uint32_t) * n, c->stream));
@@ -343,7 +345,7 @@ void free_all(lps_ctx* c) {
     dfree(c->keysA); dfree(c->valsA); dfree(c->keysB); dfree(c->valsB);
     dfree(c->skeys); dfree(c->perm); dfree(c->hist); dfree(c->digit_total); dfree(c->bbox_part);
     dfree(c->posrho); dfree(c->force); dfree(c->fp); dfree(c->press); dfree(c->q); dfree(c->rho_pred);
-    dfree(c->mk_mask); dfree(c->mk_ok);
+    dfree(c->mk_mask); dfree(c->mk_ok); dfree(c->mk_cq);
     dfree(c->run_end); dfree(c->cell_lb); dfree(c->block_max);
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
     dfree(c->d_mlp); dfree(c->d_mlp_packed); dfree(c->feat); dfree(c->seed); dfree(c->coll_x); dfree(c->coll_t);
@@ -381,6 +383,7 @@ lps_dev::PhysArgs phys_args(lps_ctx* c, const SimConst& k) {
     a.mk.mask = c->mk_mask;
     a.mk.ok = c->mk_ok;
     a.mk.stride = c->cap_n;
+    a.mk.cq = c->mk_cq;
     a.c = k;
     return a;
 }

@@ -34,6 +34,8 @@ struct NbMasks {
     uint32_t* mask;    // [LPS_MC_ROWS][stride]
     uint32_t* ok;      // [stride] 1: recorded by the last recording pass
     uint32_t stride;   // slots (LPS_MC_ROWS * stride < 2^32)
+    uint2* cq;         // [stride + LPS_STENCIL_PAD] predictions in 16-bit fixed point, written with each
+                       // prediction and read by the predicted-density pass's filter
 };
 
 struct ReorderArgs {

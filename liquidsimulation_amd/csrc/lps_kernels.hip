@@ -171,6 +171,61 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
 // in the same order.
 enum : int { MC_LIVE = 0, MC_RECORD = 1, MC_REPLAY = 2 };
 
+// Compact positions for the recording pass's filter.  Each prediction is also
+// stored as three 16-bit fixed-point coordinates, u = floor((x + off) * Q)
+// mod 2^16 with Q = 4096 / cell size (8 bytes: x | y << 16, z), so that pass
+// 1 of the predicted-density pass reads 8 bytes per candidate instead of 12
+// and tests it with packed 16-bit differences and two dot2 instructions.  The
+// filter only has to keep every neighbour (pass 2 decides exactly):
+// * differences wrap mod 2^16, so they are exact while |dx| < 8 cells; a
+//   neighbour pair (d <= h <= cell size) differs by at most 4096 units plus
+//   rounding on each axis, and a wrapped difference can only add candidates;
+// * for |(x + off) * Q| < 2^22 the stored value is within (-1.5, 0.5) of the
+//   exact one (floor, two roundings of at most 0.25), so a difference is off
+//   by less than 2 units per axis and a distance by less than 3.5;
+// * the test keeps s_q <= (ceil(sqrt(s_max) * Q) + 8)^2, so every pair with
+//   s2 <= s_max is kept;
+// * a particle whose own |(x + off) * Q| reaches 2^21 on some axis filters
+//   with floats instead; its neighbours then all lie within 2^22.
+// Far predictions (beyond 2^22) store a saturated value that can only add
+// candidates.  s_q is at most 3 * 2^30 and is compared unsigned.
+struct CompactFilter {
+    const uint2* cq;  // nullptr: float filter
+    uint2 qi;         // this particle's entry
+    uint32_t T;       // squared radius in units
+};
+
+typedef short lps_short2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ float compact_scale(const GridView& G) { return 4096.f / G.gs; }
+
+__device__ __forceinline__ uint32_t quantize1(float x, float off, float qs) {
+    const float v = fminf(fmaxf((x + off) * qs, -0x1p30f), 0x1p30f);
+    return (uint32_t)(int)floorf(v) & 0xffffu;
+}
+
+__device__ __forceinline__ uint2 quantize(float x, float y, float z, const GridView& G, float qs) {
+    return make_uint2(quantize1(x, G.off0, qs) | (quantize1(y, G.off1, qs) << 16), quantize1(z, G.off2, qs));
+}
+
+// Whether particle (x, y, z) may use the compact filter.
+__device__ __forceinline__ bool compact_near(float x, float y, float z, const GridView& G, float qs) {
+    return fabsf((x + G.off0) * qs) < 0x1p21f && fabsf((y + G.off1) * qs) < 0x1p21f &&
+           fabsf((z + G.off2) * qs) < 0x1p21f;
+}
+
+__device__ __forceinline__ uint32_t compact_T(float s_max, float qs) {
+    const float r = ceilf(sqrtf(s_max) * qs) + 8.f;
+    return (uint32_t)(r * r);
+}
+
+__device__ __forceinline__ bool compact_keep(const uint2& e, const uint2& qi, uint32_t T) {
+    const lps_short2 dxy = __builtin_bit_cast(lps_short2, e.x) - __builtin_bit_cast(lps_short2, qi.x);
+    const lps_short2 dz = __builtin_bit_cast(lps_short2, e.y) - __builtin_bit_cast(lps_short2, qi.y);
+    const int sq = __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, 0, false), false);
+    return (uint32_t)sq <= T;
+}
+
 template <class Acc, class Pre, class F>
 __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px,
                                              float py, float pz, float s_max, Acc& accept_extra, Pre& pre, F& fn) {
@@ -212,8 +267,9 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
                                                      float ay, float az, const float4* __restrict__ xs, float px,
                                                      float py, float pz, float s_max, float s_relax,
                                                      const lps_dev::NbMasks& M, uint32_t i, int mode,
-                                                     Acc&& accept_extra, Pre&& pre, F&& fn) {
+                                                     const CompactFilter& CF, Acc&& accept_extra, Pre&& pre, F&& fn) {
     const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
+    const char* __restrict__ cb = reinterpret_cast<const char*>(CF.cq);
     const int xg = (int)truncf((ax + G.off0) / G.gs);
     const int yg = (int)truncf((ay + G.off1) / G.gs);
     const int zg = (int)truncf((az + G.off2) / G.gs);
@@ -243,15 +299,28 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
             for (; w0 < e; w0 += 32u) {
                 const uint32_t n = min(32u, e - w0);
                 uint32_t mask = 0u;
-                for (uint32_t u = 0; u < n; u += 8u) {
-                    const char* p8 = xb + ((w0 + u) << 4);
-                    float4 q[8];
+                if (cb) {
+                    for (uint32_t u = 0; u < n; u += 8u) {
+                        const char* p8 = cb + ((w0 + u) << 3);
+                        uint2 e[8];
 #pragma unroll
-                    for (int t = 0; t < 8; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
-                    uint32_t m8 = 0u;
+                        for (int t = 0; t < 8; t++) e[t] = *reinterpret_cast<const uint2*>(p8 + 8 * t);
+                        uint32_t m8 = 0u;
 #pragma unroll
-                    for (int t = 0; t < 8; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
-                    mask |= m8 << u;
+                        for (int t = 0; t < 8; t++) m8 |= (compact_keep(e[t], CF.qi, CF.T) ? 1u : 0u) << t;
+                        mask |= m8 << u;
+                    }
+                } else {
+                    for (uint32_t u = 0; u < n; u += 8u) {
+                        const char* p8 = xb + ((w0 + u) << 4);
+                        float4 q[8];
+#pragma unroll
+                        for (int t = 0; t < 8; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
+                        uint32_t m8 = 0u;
+#pragma unroll
+                        for (int t = 0; t < 8; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
+                        mask |= m8 << u;
+                    }
                 }
                 mask &= n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u;
                 if (w0 == b) first = mask;
@@ -907,6 +976,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     const float4 xi = pos[i];
     float sum = 0.f;
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
+                         CompactFilter{},
                          [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                          [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
     const float rho = c.mass * sum;
@@ -953,7 +1023,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     float nb = 0.f, dv = 0.f;  // warm-start features: neighbour count, velocity divergence
     // the density pass recorded this stencil's masks at the same positions
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                         MC_REPLAY, [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
+                         MC_REPLAY, CompactFilter{}, [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
                          [&](uint32_t j, const float4& xj, float s2, const float4& vj, const float3&) {
         const uint32_t idx = table_index(s2, sT, c.inv_div);
         const float rj = xj.w;
@@ -1017,6 +1087,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
         const float4 xp = predict(xo, vi, vh[i], first[i] != 0, fx + 0.f, fy + 0.f, fz + 0.f, ri, c, boxes, spheres,
                                   planes);
         xpred[i] = xp;
+        mk.cq[i] = quantize(xp.x, xp.y, xp.z, G, compact_scale(G));
         if constexpr (FEAT) {
             const float ex = xp.x - xo.x, ey = xp.y - xo.y, ez = xp.z - xo.z;
             feat[(size_t)i * LPS_FEAT + 10] = sqrtf(ex * ex + ey * ey + ez * ez) / c.h;
@@ -1047,7 +1118,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                         MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+                         MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
                          [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
         const float d = sqrt_rn(s2);
         float ux, uy, uz;
@@ -1116,9 +1187,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         const float4 xa = pos[i];
         const float4 xi = xpred[i];
         float sum = 0.f;
+        const float qs = compact_scale(G);
+        CompactFilter cf{};
+        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_T(c.s_max, qs)};
         // recorded for the pressure pass of this correction (same positions and cells)
         for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                             MC_RECORD, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+                             MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                              [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
         const float rp = c.mass * sum;
         float e = rp - c.rest_density;
@@ -1272,7 +1346,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     // the predicted-density pass of this correction recorded the masks
     for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                         MC_REPLAY, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+                         MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
                          [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
         const float d = sqrt_rn(s2);
         float ux, uy, uz;
@@ -1288,8 +1362,10 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     fp[i] = f4;
     const float4 f = force[i];
     const float4 xr = posrho[i];
-    xpred_next[i] = predict(make_float4(xr.x, xr.y, xr.z, 0.f), vel[i], vh[i], first[i] != 0, f.x + f4.x,
-                            f.y + f4.y, f.z + f4.z, xr.w, c, boxes, spheres, planes);
+    const float4 xn = predict(make_float4(xr.x, xr.y, xr.z, 0.f), vel[i], vh[i], first[i] != 0, f.x + f4.x,
+                              f.y + f4.y, f.z + f4.z, xr.w, c, boxes, spheres, planes);
+    xpred_next[i] = xn;
+    mk.cq[i] = quantize(xn.x, xn.y, xn.z, G, compact_scale(G));
 }
 
 // integrate (SPHKernels.cl:424-455) + pciAddPressureForce (SPHKernels.cl:880-890)
