@@ -37,11 +37,20 @@ constexpr int OFF_W3 = OFF_B2 + HID;
 constexpr int OFF_B3 = OFF_W3 + HID;
 static_assert(OFF_B3 + 1 == LPS_MLP_PARAMS, "parameter layout");
 
-__device__ __forceinline__ unsigned short f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (unsigned short)0x7FC0u;  // NaN stays NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+// Round to nearest even bfloat16 (v_cvt_pk_bf16_f32; equal to rounding the bit
+// pattern for every non-NaN float, a NaN stays a NaN: tests/bf16_check.hip).
+__device__ __forceinline__ unsigned short f2bf(float f) { return __builtin_bit_cast(unsigned short, (__bf16)f); }
+
+typedef float f32x2v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2v __attribute__((ext_vector_type(2)));
+
+// Two roundings in one instruction: a in the low half, b in the high half.
+__device__ __forceinline__ uint32_t f2bf2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2v{a, b}, bf16x2v));
+}
+__device__ __forceinline__ bf16x4 f2bf4(float a, float b, float c, float d) {
+    return __builtin_bit_cast(bf16x4, u32x2v{f2bf2(a, b), f2bf2(c, d)});
 }
 __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ __forceinline__ float rbf(float f) { return bf2f(f2bf(f)); }
@@ -147,27 +156,35 @@ static_assert(sizeof(MlpPacked) % 16 == 0, "packed weights are copied as uint4")
 // Inference of one group of 16 samples (column l%16), the output only: the
 // arithmetic of forward() (same products, order and roundings) without
 // keeping the pre-activations and hidden tiles the gradient needs.
-__device__ __forceinline__ float infer(const MlpPacked& S, const float* __restrict__ xrow, bool valid) {
+// xin holds features 4*(l/16)..+3 of the lane's sample.
+__device__ __forceinline__ float infer(const MlpPacked& S, const float4& xin, bool valid) {
     const uint32_t l = threadIdx.x & 63u, g = l >> 4, r16 = l & 15u;
-    bf16x4 x;
+    const float xv[4] = {xin.x, xin.y, xin.z, xin.w};
+    float v[4];
     for (int r = 0; r < 4; r++) {
         const uint32_t f = 4u * g + (uint32_t)r;
-        const float v = valid ? (xrow[f] - S.mean[f]) * S.istd[f] : 0.f;
-        x[r] = (short)f2bf(v);
+        v[r] = valid ? (xv[r] - S.mean[f]) * S.istd[f] : 0.f;
     }
+    const bf16x4 x = f2bf4(v[0], v[1], v[2], v[3]);
     bf16x4 h1[4];
     for (int mt = 0; mt < 4; mt++) {
         const f32x4 y = mfma(load4(&S.w1[(mt * 16 + r16) * IN + 4 * g]), x, f32x4{0.f, 0.f, 0.f, 0.f});
-        for (int r = 0; r < 4; r++) h1[mt][r] = (short)f2bf(fmaxf(y[r] + S.b1[mt * 16 + 4 * g + r], 0.f));
+        const float* b1 = &S.b1[mt * 16 + 4 * g];
+        h1[mt] = f2bf4(fmaxf(y[0] + b1[0], 0.f), fmaxf(y[1] + b1[1], 0.f), fmaxf(y[2] + b1[2], 0.f),
+                       fmaxf(y[3] + b1[3], 0.f));
     }
     float part = 0.f;
     for (int mt = 0; mt < 4; mt++) {
         f32x4 acc{0.f, 0.f, 0.f, 0.f};
         for (int kt = 0; kt < 4; kt++) acc = mfma(load4(&S.w2[(mt * 16 + r16) * HID + kt * 16 + 4 * g]), h1[kt], acc);
-        for (int r = 0; r < 4; r++) {
-            const float h2 = bf2f(f2bf(fmaxf(acc[r] + S.b2[mt * 16 + 4 * g + r], 0.f)));
-            part += S.w3[mt * 16 + 4 * g + r] * h2;
-        }
+        const float* b2 = &S.b2[mt * 16 + 4 * g];
+        const float* w3 = &S.w3[mt * 16 + 4 * g];
+        const uint32_t lo = f2bf2(fmaxf(acc[0] + b2[0], 0.f), fmaxf(acc[1] + b2[1], 0.f));
+        const uint32_t hi = f2bf2(fmaxf(acc[2] + b2[2], 0.f), fmaxf(acc[3] + b2[3], 0.f));
+        part += w3[0] * __uint_as_float(lo << 16);
+        part += w3[1] * __uint_as_float(lo & 0xFFFF0000u);
+        part += w3[2] * __uint_as_float(hi << 16);
+        part += w3[3] * __uint_as_float(hi & 0xFFFF0000u);
     }
     part += __shfl_xor(part, 16, 64);
     part += __shfl_xor(part, 32, 64);
@@ -210,7 +227,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(const MlpPacked* __re
     for (uint32_t bb = blockIdx.x * LPS_BLOCK; bb < n; bb += gridDim.x * LPS_BLOCK)
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
-            const float v = infer(S, x + (size_t)min(p, n - 1u) * IN, p < n);
+            const float4 xin = *reinterpret_cast<const float4*>(x + (size_t)min(p, n - 1u) * IN + 4u * (l >> 4));
+            const float v = infer(S, xin, p < n);
             if (p < n && (l >> 4) == 0u) y[p] = v;
         }
 }
@@ -233,15 +251,22 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restr
     load_packed(S, packed);
     __syncthreads();
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
-    // one 256-slot chunk is one particle block, so it belongs to one scene;
-    // chunks of scenes that stopped at the first correction are skipped
-    for (uint32_t bb = blockIdx.x * LPS_BLOCK; bb < n; bb += gridDim.x * LPS_BLOCK) {
-        const uint32_t s = sc.block_scene[bb / LPS_BLOCK];
-        if (sc.grid[s].stop_iter != 0u) continue;
+    // one block per 256-slot chunk, which is one particle block and so belongs
+    // to one scene; chunks of scenes that stopped at the first correction are
+    // skipped.  The wave's four groups of 16 samples load their features first.
+    const uint32_t bb = blockIdx.x * LPS_BLOCK;
+    {
+        const uint32_t s = sc.block_scene[blockIdx.x];
+        if (sc.grid[s].stop_iter != 0u) return;
         const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
+        float4 xin[4];
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
-            const float y = infer(S, feat + (size_t)min(p, n - 1u) * IN, p < end);
+            xin[grp] = *reinterpret_cast<const float4*>(feat + (size_t)min(p, n - 1u) * IN + 4u * (l >> 4));
+        }
+        for (uint32_t grp = 0; grp < 4; grp++) {
+            const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
+            const float y = infer(S, xin[grp], p < end);
             if (p < end && (l >> 4) == 0u) {
                 float v = y * tscale * pscale;
                 if (!isfinite(v)) v = 0.f;
@@ -457,8 +482,10 @@ void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed) {
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t n, float* press,
                      float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
                      const Scenes& sc) {
-    k_mlp_seed<<<mlp_grid(n), LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale, feat, n, press, q,
-                                                 seed, rho_pred, pscale, clamp_nonneg, g, sc);
+    // one block per particle block (n is a whole number of blocks)
+    k_mlp_seed<<<(n + LPS_BLOCK - 1u) / LPS_BLOCK, LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale,
+                                                                      feat, n, press, q, seed, rho_pred, pscale,
+                                                                      clamp_nonneg, g, sc);
 }
 
 void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y) {

@@ -9,7 +9,7 @@ ROOT = os.path.dirname(HERE)
 BIN = os.path.join(HERE, "_bin")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-CHECKS = {"div_rn_check": "div_rn_check.hip"}
+CHECKS = {"div_rn_check": "div_rn_check.hip", "bf16_check": "bf16_check.hip"}
 
 
 def path(name: str) -> str:
