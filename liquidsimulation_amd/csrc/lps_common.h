@@ -71,6 +71,7 @@ struct DevGrid {
     uint32_t iterations;     // corrections run this step (max over scenes)
     uint32_t final_err;      // max over scenes of the last correction's max |e| (float bits)
     uint32_t converged;      // every scene met the tolerance
+    uint32_t tiny_pred;      // a prediction of this step has a coordinate in (0, 2^-56)
     // Correction kernels run only the particle blocks of scenes whose loop
     // is still running.  Two block lists alternate; list_of[k % 64] names the
     // list of correction k (both of its passes).  Entries are written once per

@@ -15,6 +15,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 
 #include "lps_common.h"
 #include "lps_div.h"
@@ -509,17 +510,26 @@ __device__ __forceinline__ bool step_dead(const DevGrid* g) { return g->status !
 // numerator into +0 as in the reference; the IEEE division elsewhere gives
 // -0 for it, and a -0 quotient adds nothing different to the force sums,
 // which start at +0 and so never hold -0.  `fastdiv` is the caller's uniform
-// bound d < 2^40.  Two distinct particles at exactly the same point (d == 0;
+// bound d < 2^40; NOCHECK says every coordinate is known to be 0 or >= 2^-56
+// in magnitude (the numerators, their differences, then need no check, and
+// the caller also guarantees the bound).  Two distinct particles at exactly the same point (d == 0;
 // the box collider snaps every particle crossing three walls onto one corner
 // point) would give 0/0 in the reference; the pair gets the antisymmetric
 // unit direction +y for the higher sorted index and -y for the lower one,
 // which separates it.
+template <bool NOCHECK>
 __device__ __forceinline__ void pair_dir(const float3& dl, float d, uint32_t i, uint32_t j, bool fastdiv, float& ux,
                                          float& uy, float& uz) {
     if (d > 0.f) {
         const float ax = -dl.x, ay = -dl.y, az = -dl.z;
-        const uint32_t key = min(min(lps_div::num_key(ax), lps_div::num_key(ay)), lps_div::num_key(az));
-        if (fastdiv && lps_div::div_rn_ok(d, key)) {
+        bool fast;
+        if constexpr (NOCHECK) {
+            fast = lps_div::div_rn_ok_d(d);
+        } else {
+            const uint32_t key = min(min(lps_div::num_key(ax), lps_div::num_key(ay)), lps_div::num_key(az));
+            fast = fastdiv && lps_div::div_rn_ok(d, key);
+        }
+        if (fast) {
             const float y = lps_div::rcp_rn(d);
             ux = lps_div::div_rn(ax, d, y);
             uy = lps_div::div_rn(ay, d, y);
@@ -676,6 +686,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::
     g->iterations = 0u;
     g->final_err = 0u;
     g->converged = 0u;
+    g->tiny_pred = 0u;
     g->act_count[0] = S.nblocks;
     g->act_count[1] = 0u;
     g->list_of[1] = 0u;
@@ -993,7 +1004,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
 template <bool FEAT>
 __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
-    const uint8_t* __restrict__ first, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
+    const uint8_t* __restrict__ first, DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
     const lps_plane* __restrict__ planes0, float4* __restrict__ force, float4* __restrict__ xpred,
     const float* __restrict__ pprev, float* __restrict__ feat, lps_dev::Scenes S, lps_dev::NbMasks mk,
@@ -1088,6 +1099,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
                                   planes);
         xpred[i] = xp;
         mk.cq[i] = quantize(xp.x, xp.y, xp.z, G, compact_scale(G));
+        if (lps_div::tiny_coord(xp.x, xp.y, xp.z)) g->tiny_pred = 1u;
         if constexpr (FEAT) {
             const float ex = xp.x - xo.x, ey = xp.y - xo.y, ez = xp.z - xo.z;
             feat[(size_t)i * LPS_FEAT + 10] = sqrtf(ex * ex + ey * ey + ez * ez) / c.h;
@@ -1122,7 +1134,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
                          [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
         const float d = sqrt_rn(s2);
         float ux, uy, uz;
-        pair_dir(dl, d, i, j, fastdiv, ux, uy, uz);
+        pair_dir<false>(dl, d, i, j, fastdiv, ux, uy, uz);
         const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
         const float sc = tf + qj;
         pfx += ux * sc * w;
@@ -1315,7 +1327,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
 // prediction (pciIntegrate + pciHandleCollisions) from f + fp.
 __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, const float* __restrict__ rho_pred,
-    const float* __restrict__ q, const DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
+    const float* __restrict__ q, DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
@@ -1342,21 +1354,31 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4 xa = pos[i];
     const float4 xi = xpred[i];
     const float tf = q[i];
-    const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
+    // accepted pairs have d < 2^40; the predictions' coordinates were checked
+    // when they were written (one instantiation of the stencil per case, so
+    // the pair loop carries no mode test)
+    const bool fastdiv = c.s_max < 0x1p78f;
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    // the predicted-density pass of this correction recorded the masks
-    for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                         MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
-                         [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
-        const float d = sqrt_rn(s2);
-        float ux, uy, uz;
-        pair_dir(dl, d, i, j, fastdiv, ux, uy, uz);
-        const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
-        const float sc = tf + qj;
-        pfx += ux * sc * w;
-        pfy += uy * sc * w;
-        pfz += uz * sc * w;
-    });
+    auto pressure = [&](auto nocheck) {
+        // the predicted-density pass of this correction recorded the masks
+        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                             MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; },
+                             [&](uint32_t j) { return q[j]; },
+                             [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
+            const float d = sqrt_rn(s2);
+            float ux, uy, uz;
+            pair_dir<decltype(nocheck)::value>(dl, d, i, j, fastdiv, ux, uy, uz);
+            const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
+            const float sc = tf + qj;
+            pfx += ux * sc * w;
+            pfy += uy * sc * w;
+            pfz += uz * sc * w;
+        });
+    };
+    if (fastdiv && g->tiny_pred == 0u)
+        pressure(std::true_type{});
+    else
+        pressure(std::false_type{});
     const float ri = rho_pred[i];
     const float4 f4 = make_float4((-pfx) * c.mass * ri, (-pfy) * c.mass * ri, (-pfz) * c.mass * ri, 0.f);
     fp[i] = f4;
@@ -1366,6 +1388,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
                               f.y + f4.y, f.z + f4.z, xr.w, c, boxes, spheres, planes);
     xpred_next[i] = xn;
     mk.cq[i] = quantize(xn.x, xn.y, xn.z, G, compact_scale(G));
+    if (lps_div::tiny_coord(xn.x, xn.y, xn.z)) g->tiny_pred = 1u;
 }
 
 // integrate (SPHKernels.cl:424-455) + pciAddPressureForce (SPHKernels.cl:880-890)

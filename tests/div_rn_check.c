@@ -4,8 +4,9 @@
  * reciprocal's documented accuracy), one fused Newton step
  *     e = 1 - d*y0,  y = y0 + e*y0
  * must give the correctly rounded 1.0f/d, except when d's significand is all
- * ones (those failures are counted apart: the guard sends such d to IEEE
- * division).  Also draws random quotients a/d
+ * ones and y0 is the lower candidate (counted apart; the device program
+ * div_rn_check.hip shows the hardware estimate is never that one).  Also draws
+ * random quotients a/d
  * (a and d in [1, 2)) and checks the Markstein correction built on that
  * reciprocal against the IEEE quotient; the device program div_rn_check.hip
  * covers every such pair.  The uncorrected product a * (1/d) must miss some of

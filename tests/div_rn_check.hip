@@ -1,9 +1,9 @@
 // Device check of the shared-reciprocal division (liquidsimulation_amd/csrc/lps_div.h)
 // against the IEEE quotient, with the hardware reciprocal estimate:
-//   div_rn_check rcp                  every d in [2^-78, 2^40) whose significand is
-//                                     not all ones: rcp_rn(d) == 1.0f / d
+//   div_rn_check rcp                  every d in [2^-78, 2^40):
+//                                     rcp_rn(d) == 1.0f / d
 //   div_rn_check pairs A0 A1          every significand pair a = 1.m_a, d = 1.m_d
-//                                     with m_a in [A0, A1) and m_d not all ones:
+//                                     with m_a in [A0, A1):
 //                                     div_rn(a, d, rcp_rn(d)) == a / d
 // (significands as 23-bit integers; pairs 0 8388608 is the exhaustive run).
 // Prints the number of cases and of mismatches, the first mismatch if any, and
@@ -29,7 +29,6 @@
     } while (0)
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kAllOnes = 0x7fffffu;
 
 __device__ __forceinline__ uint32_t block_sum(uint32_t v) {
     __shared__ uint32_t s[kBlock / 64];
@@ -50,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_rcp(uint32_t u0, uint32_t count, uin
     if (t < count) {
         const uint32_t u = u0 + t;
         const float d = __uint_as_float(u);
-        if ((u & kAllOnes) != kAllOnes && lps_div::rcp_rn(d) != 1.f / d) {
+        if (lps_div::rcp_rn(d) != 1.f / d) {
             nbad = 1;
             f = u;
         }
@@ -66,14 +65,12 @@ __global__ __launch_bounds__(kBlock) void k_pairs(uint32_t a0, uint32_t na, uint
     const float y = lps_div::rcp_rn(d);
     uint32_t nbad = 0;
     uint64_t f = 0;
-    if (md != kAllOnes) {
-        for (uint32_t k = 0; k < na; k++) {
-            const uint32_t ua = 0x3f800000u | (a0 + k);
-            const float a = __uint_as_float(ua);
-            if (lps_div::div_rn(a, d, y) != a / d) {
-                if (nbad == 0) f = ((uint64_t)ua << 32) | __float_as_uint(d);
-                nbad++;
-            }
+    for (uint32_t k = 0; k < na; k++) {
+        const uint32_t ua = 0x3f800000u | (a0 + k);
+        const float a = __uint_as_float(ua);
+        if (lps_div::div_rn(a, d, y) != a / d) {
+            if (nbad == 0) f = ((uint64_t)ua << 32) | __float_as_uint(d);
+            nbad++;
         }
     }
     const uint32_t n = block_sum(nbad);
@@ -153,7 +150,7 @@ int main(int argc, char** argv) {
         std::vector<uint64_t> first(nblk);
         CK(hipMemcpy(bad.data(), dbad, nblk * sizeof(uint32_t), hipMemcpyDeviceToHost));
         CK(hipMemcpy(first.data(), dfirst, nblk * sizeof(uint64_t), hipMemcpyDeviceToHost));
-        return report("quotients", (uint64_t)(a1 - a0) * ((1u << 23) - 1u), bad, first);
+        return report("quotients", (uint64_t)(a1 - a0) << 23, bad, first);
     }
     std::fprintf(stderr, "usage: div_rn_check rcp | pairs A0 A1\n");
     return 2;

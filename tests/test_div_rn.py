@@ -3,12 +3,12 @@ one shared reciprocal and Markstein's correction (liquidsimulation_amd/csrc/lps_
 where that is exact, and with IEEE division elsewhere.  This pins the method:
 
 * CPU: for every d in [2^-78, 2^40) and every reciprocal estimate within one
-  ulp of 1/d, the Newton step gives the correctly rounded 1/d except for the
-  all-ones significand (which the guard excludes); random quotients match
-  IEEE division (C, OpenMP, a few seconds).
-* GPU: with the hardware estimate, the reciprocal of every such d equals
-  1.0f/d, and the corrected quotient equals the IEEE quotient for slices of
-  significand pairs.  The exhaustive run over all 2^46 pairs is
+  ulp of 1/d, the Newton step gives the correctly rounded 1/d, except for an
+  all-ones significand with the lower estimate; random quotients match IEEE
+  division (C, OpenMP, a few seconds).
+* GPU: with the hardware estimate (never that lower one), the reciprocal of
+  every such d equals 1.0f/d, and the corrected quotient equals the IEEE
+  quotient for slices of significand pairs.  The exhaustive run over all 2^46 pairs is
   `tests/_bin/div_rn_check pairs 0 8388608` (about a minute on one MI355X).
 """
 import os
