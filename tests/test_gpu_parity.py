@@ -85,6 +85,27 @@ def test_multi_step_state_matches(gpu, oracle, method):
     assert ulp_diff(p["v"], a.v) == 0
 
 
+def test_tiny_coordinates_take_the_checked_division(gpu, oracle):
+    """Predictions with a coordinate in (0, 2^-56) switch the pressure pass to
+    the quotient guard that checks every numerator.  The scene is shifted so
+    that z = 0 lies mid-box; two particles far from the block sit at z = 1e-30
+    and 3e-30 (no z velocity, z forces of order 1e-28), so their predictions
+    keep a tiny z and their mutual difference is tiny too (IEEE division)."""
+    import dataclasses
+    sc = scenes.small_block(10, 12, 9, method=METHOD_PCISPH)
+    b = sc.colliders.boxes[0]
+    sc.colliders.boxes = [dataclasses.replace(b, center=(b.center[0], b.center[1], 0.0))]
+    x = sc.x.copy()
+    x[:, 2] -= np.float32(b.center[2])
+    extra = np.array([[0.7, 0.3, 1e-30], [0.7, 0.3 + 0.5 * sc.params.particle_radius, 3e-30]], np.float32)
+    sc.x = np.concatenate([x, extra]).astype(np.float32)
+    st = oracle.State.from_scene(sc.x)
+    s, _, a = _compare_step(oracle, sc, st)
+    z = np.abs(a.x[:, 2])
+    assert 0.0 < float(np.min(z[z > 0])) < 2.0 ** -56  # the predictions kept a tiny coordinate
+    s.close()
+
+
 def test_obstacles_single_step(gpu, oracle):
     """Spheres (obstacle + boundary) and a plane in a small PCISPH scene."""
     from liquidsimulation_amd.abi import Sphere
