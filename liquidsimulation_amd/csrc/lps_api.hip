@@ -96,7 +96,7 @@ struct lps_ctx {
     bool capture = false;         // compute features every PCISPH step (for lps_dump_pairs)
     float* d_mlp = nullptr;       // params [LPS_MLP_PARAMS] then fscale [2 * LPS_MLP_IN]
     uint8_t* d_mlp_packed = nullptr;  // BF16 inference copy of d_mlp (lps_dev::mlp_packed_bytes())
-    float* feat = nullptr;        // [cap_n][LPS_MLP_IN] features of the current step
+    float* feat = nullptr;        // [LPS_MLP_IN][cap_n] features of the current step (feature-major)
     float* seed = nullptr;        // [cap_n] seeded pressure of the current step
     bool feat_valid = false;      // the last step wrote features (and final pressure)
     bool seed_valid = false;      // the last step applied the seed
@@ -525,8 +525,8 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     if (kk == 1u && seeded) {
         // the first correction's pressure update is replaced by the seed
         Timed t(c, "mlp_seed");
-        lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->n, c->press, c->q, c->seed, c->rho_pred,
-                                 delta * c->params.rest_density,
+        lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->cap_n, c->n, c->press, c->q,
+                                 c->seed, c->rho_pred, delta * c->params.rest_density,
                                  c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c));
     }
     {
@@ -652,7 +652,7 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
             const uint32_t n = si.count;
             uint64_t count = n > off ? ((uint64_t)(n - off) + c->coll_stride - 1u) / c->coll_stride : 0u;
             count = std::min<uint64_t>(count, c->coll_cap - c->coll_rows);
-            lps_dev::launch_collect(s, c->feat + (size_t)si.slot0 * LPS_MLP_IN, c->press + si.slot0,
+            lps_dev::launch_collect(s, c->feat + si.slot0, c->cap_n, c->press + si.slot0,
                                     delta * c->params.rest_density, n, c->coll_stride, off, (uint32_t)count,
                                     c->coll_x, c->coll_t, c->coll_rows);
             c->coll_rows += count;
@@ -1495,7 +1495,12 @@ int lps_dump_pairs(lps_ctx* c, float* features, float* target, float* seed, uint
     const uint32_t ns = c->n;
     std::vector<float> f((size_t)ns * LPS_MLP_IN);
     if (features) {
-        HIPC(hipMemcpy(f.data(), c->feat, sizeof(float) * LPS_MLP_IN * (size_t)ns, hipMemcpyDeviceToHost));
+        // feature-major on the device: one row of ns slots per feature
+        std::vector<float> col(ns);
+        for (uint32_t k = 0; k < LPS_MLP_IN; k++) {
+            HIPC(hipMemcpy(col.data(), c->feat + (size_t)k * c->cap_n, sizeof(float) * ns, hipMemcpyDeviceToHost));
+            for (uint32_t i = 0; i < ns; i++) f[(size_t)i * LPS_MLP_IN + k] = col[i];
+        }
         compact_slots(c, f, features, LPS_MLP_IN);
     }
     if (target) {

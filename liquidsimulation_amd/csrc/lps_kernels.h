@@ -113,9 +113,10 @@ struct RowMap {
 // p_i = pscale * tscale * MLP(features_i) (non-negative when negative
 // pressures are disabled, 0 when not finite), q_i = p_i / rho*_i^2,
 // seed_i = p_i.  No-op once the step has stopped or failed.
-void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t n, float* press,
-                     float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
-                     const Scenes& sc);
+// feat: feature-major, feature f of slot i at f * fs + i
+void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
+                     float* press, float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg,
+                     const DevGrid* g, const Scenes& sc);
 // BF16 inference weights (k_mlp_seed's LDS layout) from the FP32 master copy
 size_t mlp_packed_bytes();
 void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed);
@@ -126,8 +127,8 @@ void launch_mlp_grad(hipStream_t s, const MlpArgs& m, const float* x, const floa
 void launch_adam(hipStream_t s, float* params, float* grad, float* m1, float* m2, uint32_t count, float lr,
                  float b1, float b2, float eps, uint32_t step);
 void launch_feature_stats(hipStream_t s, const float* x, uint64_t n, uint32_t cols, double* acc);
-void launch_collect(hipStream_t s, const float* feat, const float* press, float pscale, uint32_t n, uint32_t stride,
-                    uint32_t offset, uint32_t count, float* X, float* T, uint64_t base);
+void launch_collect(hipStream_t s, const float* feat, uint32_t fs, const float* press, float pscale, uint32_t n,
+                    uint32_t stride, uint32_t offset, uint32_t count, float* X, float* T, uint64_t base);
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
                          const float* q, float4* fp, float4* xpred_next, uint32_t k);
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,

@@ -1071,17 +1071,19 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     snx *= c.mass; sny *= c.mass; snz *= c.mass;
     const float snl = sqrtf(snx * snx + sny * sny + snz * snz);
     if constexpr (FEAT) {
-        float* fr = feat + (size_t)i * LPS_FEAT;
-        fr[0] = ri / c.rest_density - 1.f;
-        fr[1] = nb * (1.f / 32.f);
-        fr[2] = dv * c.mass * c.dt;
-        fr[3] = snx * c.h;
-        fr[4] = sny * c.h;
-        fr[5] = snz * c.h;
-        fr[6] = snl * c.h;
-        fr[7] = pprev[i] / (c.delta * c.rest_density);
-        fr[8] = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * c.dt / c.h;
-        fr[9] = first[i] ? 1.f : 0.f;
+        // feature-major: feature f of slot i at f * stride + i (coalesced stores)
+        float* fr = feat + i;
+        const size_t fs = mk.stride;
+        fr[0 * fs] = ri / c.rest_density - 1.f;
+        fr[1 * fs] = nb * (1.f / 32.f);
+        fr[2 * fs] = dv * c.mass * c.dt;
+        fr[3 * fs] = snx * c.h;
+        fr[4 * fs] = sny * c.h;
+        fr[5 * fs] = snz * c.h;
+        fr[6 * fs] = snl * c.h;
+        fr[7 * fs] = pprev[i] / (c.delta * c.rest_density);
+        fr[8 * fs] = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * c.dt / c.h;
+        fr[9 * fs] = first[i] ? 1.f : 0.f;
     }
     if (snl > c.st_threshold) {
         const float k = -c.surface_tension * lc * c.mass;
@@ -1102,7 +1104,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
         if (lps_div::tiny_coord(xp.x, xp.y, xp.z)) g->tiny_pred = 1u;
         if constexpr (FEAT) {
             const float ex = xp.x - xo.x, ey = xp.y - xo.y, ez = xp.z - xo.z;
-            feat[(size_t)i * LPS_FEAT + 10] = sqrtf(ex * ex + ey * ey + ez * ez) / c.h;
+            feat[(size_t)10 * mk.stride + i] = sqrtf(ex * ex + ey * ey + ez * ez) / c.h;
         }
     }
 }
@@ -1218,13 +1220,14 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         press[i] = p;
         q[i] = p / (rp * rp);
         if (FEAT && k == 1u) {
-            float* fr = feat + (size_t)i * LPS_FEAT;
+            float* fr = feat + i;  // feature-major, as in k_forces
+            const size_t fs = mk.stride;
             const float e1 = rp / c.rest_density - 1.f;
-            fr[11] = e1;
-            fr[12] = fmaxf(e1, 0.f);
-            fr[13] = dp / (c.delta * c.rest_density);
-            fr[14] = (rp - posrho[i].w) / c.rest_density;
-            fr[15] = e1 * e1 * 100.f;
+            fr[11 * fs] = e1;
+            fr[12 * fs] = fmaxf(e1, 0.f);
+            fr[13 * fs] = dp / (c.delta * c.rest_density);
+            fr[14 * fs] = (rp - posrho[i].w) / c.rest_density;
+            fr[15 * fs] = e1 * e1 * 100.f;
         }
         ebits = __float_as_uint(fabsf(e));
     }

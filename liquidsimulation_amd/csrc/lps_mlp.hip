@@ -241,7 +241,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(const MlpPacked* __re
 // negative pressures are disabled, non-finite values replaced by 0;
 // q_i = p_i / rho*_i^2 for the pressure-force pass.
 __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restrict__ packed, float tscale,
-                                                        const float* __restrict__ feat, uint32_t n,
+                                                        const float* __restrict__ feat, uint32_t fs, uint32_t n,
                                                         float* __restrict__ press, float* __restrict__ q,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
                                                         float pscale, int clamp_nonneg, const DevGrid* g,
@@ -259,10 +259,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restr
         const uint32_t s = sc.block_scene[blockIdx.x];
         if (sc.grid[s].stop_iter != 0u) return;
         const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
+        // features are feature-major (feature f of slot p at f * fs + p)
         float4 xin[4];
         for (uint32_t grp = 0; grp < 4; grp++) {
-            const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
-            xin[grp] = *reinterpret_cast<const float4*>(feat + (size_t)min(p, n - 1u) * IN + 4u * (l >> 4));
+            const uint32_t p = min(bb + w * 64u + grp * 16u + (l & 15u), n - 1u);
+            const float* fp = feat + (size_t)(4u * (l >> 4)) * fs + p;
+            xin[grp] = make_float4(fp[0], fp[fs], fp[2 * (size_t)fs], fp[3 * (size_t)fs]);
         }
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
@@ -450,17 +452,17 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_feature_stats(const float* __rest
 }
 
 // Training-set append: particle offset + r * stride (r < count) gives row
-// base + r: its features and its final pressure divided by pscale.
-__global__ void k_collect(const float* __restrict__ feat, const float* __restrict__ press, float inv_pscale,
-                          uint32_t n, uint32_t stride, uint32_t offset, uint32_t count, float* __restrict__ X,
-                          float* __restrict__ Tg, uint64_t base) {
+// base + r: its features (feature-major source, feature f at f * fs + i) and
+// its final pressure divided by pscale.
+__global__ void k_collect(const float* __restrict__ feat, uint32_t fs, const float* __restrict__ press,
+                          float inv_pscale, uint32_t n, uint32_t stride, uint32_t offset, uint32_t count,
+                          float* __restrict__ X, float* __restrict__ Tg, uint64_t base) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= count) return;
     const uint64_t i = (uint64_t)offset + (uint64_t)r * stride;
     if (i >= n) return;
-    const float4* src = reinterpret_cast<const float4*>(feat + i * IN);
-    float4* dst = reinterpret_cast<float4*>(X + (base + r) * IN);
-    for (int f = 0; f < IN / 4; f++) dst[f] = src[f];
+    float* dst = X + (base + r) * IN;
+    for (int f = 0; f < IN; f++) dst[f] = feat[(uint64_t)f * fs + i];
     Tg[base + r] = press[i] * inv_pscale;
 }
 
@@ -479,12 +481,12 @@ void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed) {
     k_mlp_pack<<<1, LPS_BLOCK, 0, s>>>(m.params, m.fscale, static_cast<MlpPacked*>(packed));
 }
 
-void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t n, float* press,
-                     float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg, const DevGrid* g,
-                     const Scenes& sc) {
+void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
+                     float* press, float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg,
+                     const DevGrid* g, const Scenes& sc) {
     // one block per particle block (n is a whole number of blocks)
     k_mlp_seed<<<(n + LPS_BLOCK - 1u) / LPS_BLOCK, LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale,
-                                                                      feat, n, press, q, seed, rho_pred, pscale,
+                                                                      feat, fs, n, press, q, seed, rho_pred, pscale,
                                                                       clamp_nonneg, g, sc);
 }
 
@@ -507,10 +509,11 @@ void launch_feature_stats(hipStream_t s, const float* x, uint64_t n, uint32_t co
     k_feature_stats<<<256, LPS_BLOCK, 0, s>>>(x, n, cols, acc);
 }
 
-void launch_collect(hipStream_t s, const float* feat, const float* press, float pscale, uint32_t n, uint32_t stride,
-                    uint32_t offset, uint32_t count, float* X, float* T, uint64_t base) {
+void launch_collect(hipStream_t s, const float* feat, uint32_t fs, const float* press, float pscale, uint32_t n,
+                    uint32_t stride, uint32_t offset, uint32_t count, float* X, float* T, uint64_t base) {
     if (count == 0) return;
-    k_collect<<<(count + 255u) / 256u, 256, 0, s>>>(feat, press, 1.f / pscale, n, stride, offset, count, X, T, base);
+    k_collect<<<(count + 255u) / 256u, 256, 0, s>>>(feat, fs, press, 1.f / pscale, n, stride, offset, count, X, T,
+                                                    base);
 }
 
 }  // namespace lps_dev
