@@ -253,12 +253,12 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
         }
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            if ((uint32_t)t < cnt) {
-                const uint32_t j = w0 + kk[t];
-                const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
-                const float s2 = dx * dx + dy * dy + dz * dz;
-                if (s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t], make_float3(dx, dy, dz));
-            }
+            // slots past cnt repeat candidate kk[0]; one predicate keeps the
+            // exec-mask bookkeeping to a single region per candidate
+            const uint32_t j = w0 + kk[t];
+            const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
+            const float s2 = dx * dx + dy * dy + dz * dz;
+            if ((uint32_t)t < cnt && s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t], make_float3(dx, dy, dz));
         }
     }
 }
@@ -520,25 +520,24 @@ __device__ __forceinline__ bool step_dead(const DevGrid* g) { return g->status !
 template <bool NOCHECK>
 __device__ __forceinline__ void pair_dir(const float3& dl, float d, uint32_t i, uint32_t j, bool fastdiv, float& ux,
                                          float& uy, float& uz) {
-    if (d > 0.f) {
-        const float ax = -dl.x, ay = -dl.y, az = -dl.z;
-        bool fast;
-        if constexpr (NOCHECK) {
-            fast = lps_div::div_rn_ok_d(d);
-        } else {
-            const uint32_t key = min(min(lps_div::num_key(ax), lps_div::num_key(ay)), lps_div::num_key(az));
-            fast = fastdiv && lps_div::div_rn_ok(d, key);
-        }
-        if (fast) {
-            const float y = lps_div::rcp_rn(d);
-            ux = lps_div::div_rn(ax, d, y);
-            uy = lps_div::div_rn(ay, d, y);
-            uz = lps_div::div_rn(az, d, y);
-        } else {
-            ux = ax / d;
-            uy = ay / d;
-            uz = az / d;
-        }
+    // one branch on the common path (the shared reciprocal implies d > 0)
+    const float ax = -dl.x, ay = -dl.y, az = -dl.z;
+    bool fast;
+    if constexpr (NOCHECK) {
+        fast = lps_div::div_rn_ok_d(d);
+    } else {
+        const uint32_t key = min(min(lps_div::num_key(ax), lps_div::num_key(ay)), lps_div::num_key(az));
+        fast = fastdiv && lps_div::div_rn_ok(d, key);
+    }
+    if (fast) {
+        const float y = lps_div::rcp_rn(d);
+        ux = lps_div::div_rn(ax, d, y);
+        uy = lps_div::div_rn(ay, d, y);
+        uz = lps_div::div_rn(az, d, y);
+    } else if (d > 0.f) {
+        ux = ax / d;
+        uy = ay / d;
+        uz = az / d;
     } else {
         ux = 0.f;
         uy = i > j ? 1.f : -1.f;
