@@ -422,6 +422,8 @@ void enqueue_grid(lps_ctx* c) {
         r.run_end = c->run_end;
         r.skeys = c->skeys;
         r.perm = c->perm;
+        r.sc = scenes(c);
+        r.cq = c->mk_cq;
         lps_dev::launch_reorder(s, r);
         c->cur = b;
     }

@@ -34,8 +34,9 @@ struct NbMasks {
     uint32_t* mask;    // [LPS_MC_ROWS][stride]
     uint32_t* ok;      // [stride] 1: recorded by the last recording pass
     uint32_t stride;   // slots (LPS_MC_ROWS * stride < 2^32)
-    uint2* cq;         // [stride + LPS_STENCIL_PAD] predictions in 16-bit fixed point, written with each
-                       // prediction and read by the predicted-density pass's filter
+    uint2* cq;         // [stride + LPS_STENCIL_PAD] positions in 16-bit fixed point for the recording
+                       // passes' filter: the sorted positions (written by the reorder, read by the
+                       // density pass), then each prediction (read by the predicted-density pass)
 };
 
 struct ReorderArgs {
@@ -53,6 +54,8 @@ struct ReorderArgs {
     uint32_t* run_end;
     uint32_t* skeys;
     uint32_t* perm;
+    Scenes sc;            // slot -> scene (for the compact copy's grid)
+    uint2* cq;            // compact copy of pos_out for the density pass's filter
 };
 
 // Buffers shared by the physics kernels of one step (sorted order).

@@ -172,11 +172,12 @@ __device__ __forceinline__ void for_neighbors(const GridView& G, const uint32_t*
 // in the same order.
 enum : int { MC_LIVE = 0, MC_RECORD = 1, MC_REPLAY = 2 };
 
-// Compact positions for the recording pass's filter.  Each prediction is also
-// stored as three 16-bit fixed-point coordinates, u = floor((x + off) * Q)
-// mod 2^16 with Q = 4096 / cell size (8 bytes: x | y << 16, z), so that pass
-// 1 of the predicted-density pass reads 8 bytes per candidate instead of 12
-// and tests it with packed 16-bit differences and two dot2 instructions.  The
+// Compact positions for the recording passes' filter.  The sorted positions
+// (by the reorder) and each prediction are also stored as three 16-bit
+// fixed-point coordinates, u = floor((x + off) * Q) mod 2^16 with Q = 4096 /
+// cell size (8 bytes: x | y << 16, z), so that pass 1 of the density and
+// predicted-density passes reads 8 bytes per candidate instead of 12 and tests
+// it with packed 16-bit differences and two dot2 instructions.  The
 // filter only has to keep every neighbour (pass 2 decides exactly):
 // * differences wrap mod 2^16, so they are exact while |dx| < 8 cells; a
 //   neighbour pair (d <= h <= cell size) differs by at most 4096 units plus
@@ -854,7 +855,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
     const uint32_t* __restrict__ id_in, const float* __restrict__ pprev_in, float4* __restrict__ pos_out,
     float4* __restrict__ vel_out, float4* __restrict__ vh_out, uint8_t* __restrict__ first_out,
     uint32_t* __restrict__ id_out, float* __restrict__ pprev_out, uint32_t* __restrict__ run_end,
-    uint32_t* __restrict__ skeys_out, uint32_t* __restrict__ perm_out, uint32_t nlogical) {
+    uint32_t* __restrict__ skeys_out, uint32_t* __restrict__ perm_out, lps_dev::Scenes S, uint2* __restrict__ cq,
+    uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
@@ -871,7 +873,15 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
         perm_out[i] = j;
         if (i == n - 1u || keys[i + 1u] != k) run_end[k] = i + 1u;
     }
-    pos_out[i] = pos_in[j];
+    const float4 p = pos_in[j];
+    pos_out[i] = p;
+    if (!dead) {
+        // the density pass filters candidates on this copy (slots of one
+        // block belong to one scene)
+        const BlockScene B = block_scene(S, lb);
+        const GridView G = grid_view(g, S.grid + B.s);
+        cq[i] = quantize(p.x, p.y, p.z, G, compact_scale(G));
+    }
     vel_out[i] = vel_in[j];
     vh_out[i] = vh_in[j];
     first_out[i] = first_in[j];
@@ -985,9 +995,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     const GridView G = grid_view(g, S.grid + B.s);
     const float4 xi = pos[i];
     float sum = 0.f;
+    // the reorder wrote the compact copy of these positions
+    const float qs = compact_scale(G);
+    CompactFilter cf{};
+    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_T(c.s_max, qs)};
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
-                         CompactFilter{},
-                         [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+                         cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                          [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
     const float rho = c.mass * sum;
     float p = c.stiffness * (rho - c.rest_density);
@@ -1503,7 +1516,7 @@ void launch_reorder(hipStream_t s, const ReorderArgs& a) {
     k_reorder<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.keysA, a.valsA, a.keysB, a.valsB, a.n, a.g, a.pos_in, a.vel_in,
                                                  a.vh_in, a.first_in, a.id_in, a.pprev_in, a.pos_out, a.vel_out,
                                                  a.vh_out, a.first_out, a.id_out, a.pprev_out, a.run_end,
-                                                 a.skeys, a.perm, nl);
+                                                 a.skeys, a.perm, a.sc, a.cq, nl);
 }
 
 uint32_t cell_blocks(uint32_t cell_cap) { return cdiv(cell_cap + 1u, LPS_SCAN_TILE); }
