@@ -66,7 +66,6 @@ struct DevGrid {
     uint32_t status;         // lps_status of the step; kernels no-op when non-zero
     uint32_t cells_needed;   // total cells requested when status == GRID_OVERFLOW
     uint32_t steps_ok;       // steps whose grid set-up succeeded, minus failed loops (monotone)
-    uint32_t blocks_done;    // ticket counter of the last-block-done reduction
     uint32_t running;        // scenes whose correction loop has not stopped
     uint32_t iterations;     // corrections run this step (max over scenes)
     uint32_t final_err;      // max over scenes of the last correction's max |e| (float bits)

@@ -681,7 +681,6 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::
     g->key_bits = cm1 ? 32u - (uint32_t)__clz(cm1) : 0u;
     g->passes_done = 0u;
     g->status = status;
-    g->blocks_done = 0u;
     g->running = S.count;
     g->iterations = 0u;
     g->final_err = 0u;
@@ -1165,42 +1164,24 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
 // pciCalcDensityPressure (SPHKernels.cl:737-804) + the density-error maximum
 // of correction k (max_i |e_i|, e_i = rho*_i - rho0 scaled by npf when the
 // pressure update is negative; PCISPHSolver.cpp:113-121,152-156).  Cells come
-// from the current position, distances from the predicted ones.
-// The last block to finish evaluates the loop rule for correction k (1-based):
-// stop when k >= min and the ratio is below the tolerance (converged), when
-// k >= min and the maximum error has not improved on its best value for
-// `stall_limit` corrections, or at k == max; a NaN error marks the step
-// non-finite.  It writes {state, corrections, error bits, converged} to the
-// host slot (state 0 continue, 1 stopped, 2 error).
+// from the current position, distances from the predicted ones.  Each block
+// adds its maximum to the scene's error of correction k; k_loop_rule, launched
+// right after, applies the loop rule.
 template <bool FEAT>
 __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
-    float* __restrict__ q, uint32_t k, LoopRule rule, float* __restrict__ feat, const float4* __restrict__ posrho,
-    uint32_t* __restrict__ host_flag, lps_dev::Scenes S, lps_dev::NbMasks mk, uint32_t nlogical) {
+    float* __restrict__ q, uint32_t k, float* __restrict__ feat, const float4* __restrict__ posrho,
+    lps_dev::Scenes S, lps_dev::NbMasks mk, uint32_t nlogical) {
     const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
-    if (step_dead(g) || g->running == 0u) {
-        // A correction queued past the stop answers nothing: the host never
-        // waits on it, and it may run after the host has marked the same slot
-        // for the next step.  A dead step is answered (the host may be waiting
-        // on any correction of it) and the host drains the stream after it.
-        if (step_dead(g) && blockIdx.x == 0 && threadIdx.x == 0) {
-            host_flag[1] = g->iterations;
-            host_flag[2] = g->final_err;
-            host_flag[3] = g->converged;
-            __threadfence_system();  // the host polls word 0, then reads the others
-            host_flag[0] = 2u;
-        }
-        return;
-    }
+    // a correction queued past the stop, or of a dead step, does nothing
+    if (step_dead(g) || g->running == 0u) return;
     // only blocks of scenes whose loop is running are scheduled
     const uint32_t L = g->list_of[k % LPS_MAX_ITER_SLOTS];
-    const uint32_t nact = g->act_count[L];
     const uint32_t lb = active_block(g, S, L);
     if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
-    __shared__ uint32_t s_last, s_run, s_err, s_conv, s_nan, s_nb;
     const BlockScene B = block_scene(S, lb);
     const bool active = true;
     load_table(sW, tables);
@@ -1246,22 +1227,50 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     ebits = wave_max_u32(ebits);
     if (lane_id() == 0) s_max[threadIdx.x >> 6] = ebits;
     __syncthreads();
+    // the scene's maximum error of this correction; k_loop_rule reads it
+    // after the kernel boundary (no fence here: a device-scope fence writes
+    // back and invalidates this XCD's L2 under every block still running)
     if (threadIdx.x == 0) {
         uint32_t m = 0;
         for (uint32_t w = 0; w < LPS_BLOCK / 64; w++) m = max(m, s_max[w]);
         if (active && m) atomicMax(&S.grid[B.s].err_bits[slot], m);
-        __threadfence();
-        s_last = (atomicAdd(&g->blocks_done, 1u) == nact - 1u) ? 1u : 0u;
+    }
+}
+
+// The loop rule of correction k (1-based), one block, after the predicted-
+// density pass: stop a scene when k >= min and its maximum error ratio is
+// below the tolerance (converged), when k >= min and the maximum error has not
+// improved on its best value for `stall_limit` corrections, or at k == max; a
+// NaN error marks the step non-finite.  Builds the next correction's block
+// list and writes {state, corrections, error bits, converged} to the host slot
+// (state 0 continue, 1 stopped, 2 error).  A correction queued past the stop
+// answers nothing: the host never waits on it, and it may run after the host
+// has marked the same slot for the next step.  A dead step is answered (the
+// host may be waiting on any correction of it) and the host drains the stream
+// after it.
+__global__ __launch_bounds__(LPS_BLOCK) void k_loop_rule(DevGrid* g, lps_dev::Scenes S, uint32_t k, LoopRule rule,
+                                                         float rest_density, uint32_t* __restrict__ host_flag) {
+    if (step_dead(g) || g->running == 0u) {
+        if (step_dead(g) && threadIdx.x == 0) {
+            host_flag[1] = g->iterations;
+            host_flag[2] = g->final_err;
+            host_flag[3] = g->converged;
+            __threadfence_system();  // the host polls word 0, then reads the others
+            host_flag[0] = 2u;
+        }
+        return;
+    }
+    const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
+    const uint32_t L = g->list_of[k % LPS_MAX_ITER_SLOTS];
+    __shared__ uint32_t s_run, s_err, s_conv, s_nan, s_nb;
+    if (threadIdx.x == 0) {
         s_run = 0u;
         s_err = 0u;
         s_conv = 1u;
         s_nan = 0u;
     }
     __syncthreads();
-    if (!s_last) return;
-    // last block: every block's maximum is visible (fenced before its ticket);
-    // one thread per running scene applies the loop rule
-    __threadfence();
+    // one thread per scene
     for (uint32_t sidx = threadIdx.x; sidx < S.count; sidx += LPS_BLOCK) {
         SceneGrid* sg = S.grid + sidx;
         if (sg->stop_iter != 0u) {
@@ -1269,7 +1278,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
             if (!sg->converged) atomicAnd(&s_conv, 0u);
             continue;
         }
-        const uint32_t eb = atomicOr(&sg->err_bits[slot], 0u);
+        const uint32_t eb = sg->err_bits[slot];
         if (eb >= 0x7F800000u) {
             atomicOr(&s_nan, 1u);
             continue;
@@ -1281,7 +1290,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         } else {
             sg->stall += 1u;
         }
-        const float ratio = __uint_as_float(eb) / c.rest_density;
+        const float ratio = __uint_as_float(eb) / rest_density;
         bool stop = false;
         if (k >= rule.min_iter && ratio < rule.tol) {
             stop = true;
@@ -1318,7 +1327,6 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     }
     if (threadIdx.x != 0) return;
     g->list_of[(k + 1u) % LPS_MAX_ITER_SLOTS] = next;
-    g->blocks_done = 0u;
     uint32_t state = 0u;
     if (s_nan) {
         g->status = LPS_ERR_NONFINITE;
@@ -1551,8 +1559,9 @@ void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, f
                         float* q, uint32_t k, const LoopRule& rule, float* feat, uint32_t* host_flag) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = (feat && k == 1u) ? k_pci_density<true> : k_pci_density<false>;
-    kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, rule,
-                                            feat, a.posrho, host_flag, a.sc, a.mk, nl);
+    kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, feat,
+                                            a.posrho, a.sc, a.mk, nl);
+    k_loop_rule<<<1, LPS_BLOCK, 0, s>>>(a.g, a.sc, k, rule, a.c.rest_density, host_flag);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
