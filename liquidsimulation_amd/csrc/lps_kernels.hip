@@ -1255,7 +1255,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_loop_rule(DevGrid* g, lps_dev::Sc
             host_flag[1] = g->iterations;
             host_flag[2] = g->final_err;
             host_flag[3] = g->converged;
-            __threadfence_system();  // the host polls word 0, then reads the others
+            // release only: the host polls word 0, then reads the others (an
+            // acquire would also invalidate this XCD's L2 under the next pass)
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
             host_flag[0] = 2u;
         }
         return;
@@ -1342,7 +1344,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_loop_rule(DevGrid* g, lps_dev::Sc
     host_flag[1] = k;
     host_flag[2] = s_err;
     host_flag[3] = s_conv;
-    __threadfence_system();  // the host polls word 0, then reads the others
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // as above: words 1-3 before word 0
     host_flag[0] = state;
 }
 
