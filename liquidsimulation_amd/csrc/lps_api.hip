@@ -521,20 +521,20 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     if (mark) flag_slot(c, kk)[0] = FLAG_PENDING;
     {
         Timed t(c, "pci_density");
-        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, rule, feat,
-                                    c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
+        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, feat);
     }
     if (kk == 1u && seeded) {
         // the first correction's pressure update is replaced by the seed
         Timed t(c, "mlp_seed");
         lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->cap_n, c->n, c->press, c->q,
                                  c->seed, c->rho_pred, delta * c->params.rest_density,
-                                 c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c));
+                                 c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c), rule,
+                                 c->params.rest_density);
     }
     {
         Timed t(c, "pci_pressure");
         lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp, c->xpred[kk & 1u],
-                                     kk);
+                                     kk, rule, c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
     }
 }
 

@@ -93,7 +93,7 @@ void launch_density(hipStream_t s, const PhysArgs& a, float* press, float* q);
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat);
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, const LoopRule& rule, float* feat, uint32_t* host_flag);
+                        float* q, uint32_t k, float* feat);
 
 // ---- learned warm start (lps_mlp.hip)
 // Network parameters, fp32 master copy in this order: W1 [64][16], b1 [64],
@@ -117,9 +117,10 @@ struct RowMap {
 // pressures are disabled, 0 when not finite), q_i = p_i / rho*_i^2,
 // seed_i = p_i.  No-op once the step has stopped or failed.
 // feat: feature-major, feature f of slot i at f * fs + i
+// skips the scenes whose loop stops at the first correction (rule, rest_density)
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
                      float* press, float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg,
-                     const DevGrid* g, const Scenes& sc);
+                     const DevGrid* g, const Scenes& sc, const LoopRule& rule, float rest_density);
 // BF16 inference weights (k_mlp_seed's LDS layout) from the FP32 master copy
 size_t mlp_packed_bytes();
 void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed);
@@ -132,8 +133,10 @@ void launch_adam(hipStream_t s, float* params, float* grad, float* m1, float* m2
 void launch_feature_stats(hipStream_t s, const float* x, uint64_t n, uint32_t cols, double* acc);
 void launch_collect(hipStream_t s, const float* feat, uint32_t fs, const float* press, float pscale, uint32_t n,
                     uint32_t stride, uint32_t offset, uint32_t count, float* X, float* T, uint64_t base);
+// also applies the loop rule of correction k (block 0) and answers host_flag
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                         const float* q, float4* fp, float4* xpred_next, uint32_t k);
+                         const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
+                         uint32_t* host_flag);
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
                       uint32_t* bbox_part);
 // positions of the real particles, scenes back to back, to dst[r * stride + 0..2]

@@ -1165,8 +1165,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
 // of correction k (max_i |e_i|, e_i = rho*_i - rho0 scaled by npf when the
 // pressure update is negative; PCISPHSolver.cpp:113-121,152-156).  Cells come
 // from the current position, distances from the predicted ones.  Each block
-// adds its maximum to the scene's error of correction k; k_loop_rule, launched
-// right after, applies the loop rule.
+// adds its maximum to the scene's error of correction k; block 0 of the
+// pressure pass that follows applies the loop rule (loop_rule_block).
 template <bool FEAT>
 __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
@@ -1227,7 +1227,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     ebits = wave_max_u32(ebits);
     if (lane_id() == 0) s_max[threadIdx.x >> 6] = ebits;
     __syncthreads();
-    // the scene's maximum error of this correction; k_loop_rule reads it
+    // the scene's maximum error of this correction; the loop rule reads it
     // after the kernel boundary (no fence here: a device-scope fence writes
     // back and invalidates this XCD's L2 under every block still running)
     if (threadIdx.x == 0) {
@@ -1237,19 +1237,22 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     }
 }
 
-// The loop rule of correction k (1-based), one block, after the predicted-
-// density pass: stop a scene when k >= min and its maximum error ratio is
+// The loop rule of correction k (1-based), run by one whole block (block 0 of
+// the pressure pass, before its own particles) once the predicted-density
+// pass has ended: stop a scene when k >= min and its maximum error ratio is
 // below the tolerance (converged), when k >= min and the maximum error has not
 // improved on its best value for `stall_limit` corrections, or at k == max; a
 // NaN error marks the step non-finite.  Builds the next correction's block
 // list and writes {state, corrections, error bits, converged} to the host slot
-// (state 0 continue, 1 stopped, 2 error).  A correction queued past the stop
-// answers nothing: the host never waits on it, and it may run after the host
-// has marked the same slot for the next step.  A dead step is answered (the
-// host may be waiting on any correction of it) and the host drains the stream
-// after it.
-__global__ __launch_bounds__(LPS_BLOCK) void k_loop_rule(DevGrid* g, lps_dev::Scenes S, uint32_t k, LoopRule rule,
-                                                         float rest_density, uint32_t* __restrict__ host_flag) {
+// (state 0 continue, 1 stopped, 2 error).  The rest of the pressure pass reads
+// none of what it writes except stop_iter (a scene stopping at k has
+// stop_iter = k, which the pass treats like 0) and the step status.  A
+// correction queued past the stop answers nothing: the host never waits on
+// it, and it may run after the host has marked the same slot for the next
+// step.  A dead step is answered (the host may be waiting on any correction of
+// it) and the host drains the stream after it.
+__device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k, const LoopRule& rule,
+                                float rest_density, uint32_t* __restrict__ host_flag) {
     if (step_dead(g) || g->running == 0u) {
         if (step_dead(g) && threadIdx.x == 0) {
             host_flag[1] = g->iterations;
@@ -1357,7 +1360,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
     const lps_plane* __restrict__ planes0, float4* __restrict__ xpred_next, uint32_t k, lps_dev::Scenes S,
-    lps_dev::NbMasks mk, uint32_t nlogical) {
+    lps_dev::NbMasks mk, LoopRule rule, uint32_t* __restrict__ host_flag, uint32_t nlogical) {
+    if (blockIdx.x == 0) loop_rule_block(g, S, k, rule, c0.rest_density, host_flag);
     if (step_dead(g)) return;
     // the blocks of the scenes that ran this correction's predicted-density pass
     const uint32_t lb = active_block(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS]);
@@ -1558,20 +1562,20 @@ void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
 }
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, const LoopRule& rule, float* feat, uint32_t* host_flag) {
+                        float* q, uint32_t k, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = (feat && k == 1u) ? k_pci_density<true> : k_pci_density<false>;
     kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, feat,
                                             a.posrho, a.sc, a.mk, nl);
-    k_loop_rule<<<1, LPS_BLOCK, 0, s>>>(a.g, a.sc, k, rule, a.c.rest_density, host_flag);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                         const float* q, float4* fp, float4* xpred_next, uint32_t k) {
+                         const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
+                         uint32_t* host_flag) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_pci_pressure<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp,
                                                       a.force, a.posrho, a.vel, a.vh, a.first, a.boxes, a.spheres,
-                                                      a.planes, xpred_next, k, a.sc, a.mk, nl);
+                                                      a.planes, xpred_next, k, a.sc, a.mk, rule, host_flag, nl);
 }
 
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,

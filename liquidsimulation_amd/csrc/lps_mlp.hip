@@ -245,8 +245,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restr
                                                         float* __restrict__ press, float* __restrict__ q,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
                                                         float pscale, int clamp_nonneg, const DevGrid* g,
-                                                        lps_dev::Scenes sc) {
-    if (g->status != 0u || g->running == 0u) return;
+                                                        lps_dev::Scenes sc, LoopRule rule, float rest_density) {
+    if (g->status != 0u) return;
     __shared__ MlpPacked S;
     load_packed(S, packed);
     __syncthreads();
@@ -256,8 +256,15 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restr
     // skipped.  The wave's four groups of 16 samples load their features first.
     const uint32_t bb = blockIdx.x * LPS_BLOCK;
     {
+        // the seed replaces the first correction's pressure unless the scene's
+        // loop stops there (the loop rule at k = 1, which the pressure pass
+        // applies: a non-finite error, the tolerance met with min <= 1, or max
+        // <= 1; the stall rule cannot stop a loop at its first correction)
         const uint32_t s = sc.block_scene[blockIdx.x];
-        if (sc.grid[s].stop_iter != 0u) return;
+        const uint32_t eb = sc.grid[s].err_bits[0];
+        if (eb >= 0x7F800000u || (1u >= rule.min_iter && __uint_as_float(eb) / rest_density < rule.tol) ||
+            1u >= rule.max_iter)
+            return;
         const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
         // features are feature-major (feature f of slot p at f * fs + p)
         float4 xin[4];
@@ -483,11 +490,11 @@ void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed) {
 
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
                      float* press, float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg,
-                     const DevGrid* g, const Scenes& sc) {
+                     const DevGrid* g, const Scenes& sc, const LoopRule& rule, float rest_density) {
     // one block per particle block (n is a whole number of blocks)
     k_mlp_seed<<<(n + LPS_BLOCK - 1u) / LPS_BLOCK, LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale,
                                                                       feat, fs, n, press, q, seed, rho_pred, pscale,
-                                                                      clamp_nonneg, g, sc);
+                                                                      clamp_nonneg, g, sc, rule, rest_density);
 }
 
 void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y) {
