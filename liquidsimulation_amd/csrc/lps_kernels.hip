@@ -190,11 +190,15 @@ enum : int { MC_LIVE = 0, MC_RECORD = 1, MC_REPLAY = 2 };
 // * a particle whose own |(x + off) * Q| reaches 2^21 on some axis filters
 //   with floats instead; its neighbours then all lie within 2^22.
 // Far predictions (beyond 2^22) store a saturated value that can only add
-// candidates.  s_q is at most 3 * 2^30 and is compared unsigned.
+// candidates.  The dot products start from the bias -(T + 1), T the squared
+// radius in units, so a kept candidate is one whose saturated result is
+// negative: a kept pair's s_q - (T + 1) lies in [-(T + 1), -1], and larger
+// sums (up to 3 * 2^30) saturate at 2^31 - 1.  alignbit shifts each sign bit
+// into the mask, one instruction per candidate.
 struct CompactFilter {
     const uint2* cq;  // nullptr: float filter
     uint2 qi;         // this particle's entry
-    uint32_t T;       // squared radius in units
+    int bias;         // -(T + 1)
 };
 
 typedef short lps_short2 __attribute__((ext_vector_type(2)));
@@ -216,16 +220,16 @@ __device__ __forceinline__ bool compact_near(float x, float y, float z, const Gr
            fabsf((z + G.off2) * qs) < 0x1p21f;
 }
 
-__device__ __forceinline__ uint32_t compact_T(float s_max, float qs) {
+__device__ __forceinline__ int compact_bias(float s_max, float qs) {
     const float r = ceilf(sqrtf(s_max) * qs) + 8.f;
-    return (uint32_t)(r * r);
+    return -(int)(r * r) - 1;
 }
 
-__device__ __forceinline__ bool compact_keep(const uint2& e, const uint2& qi, uint32_t T) {
+// s_q + bias, saturated; negative exactly when s_q <= T.
+__device__ __forceinline__ uint32_t compact_diff(const uint2& e, const uint2& qi, int bias) {
     const lps_short2 dxy = __builtin_bit_cast(lps_short2, e.x) - __builtin_bit_cast(lps_short2, qi.x);
     const lps_short2 dz = __builtin_bit_cast(lps_short2, e.y) - __builtin_bit_cast(lps_short2, qi.y);
-    const int sq = __builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, 0, false), false);
-    return (uint32_t)sq <= T;
+    return (uint32_t)__builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, bias, true), true);
 }
 
 template <class Acc, class Pre, class F>
@@ -309,7 +313,8 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
                         for (int t = 0; t < 8; t++) e[t] = *reinterpret_cast<const uint2*>(p8 + 8 * t);
                         uint32_t m8 = 0u;
 #pragma unroll
-                        for (int t = 0; t < 8; t++) m8 |= (compact_keep(e[t], CF.qi, CF.T) ? 1u : 0u) << t;
+                        for (int t = 7; t >= 0; t--)
+                            m8 = __builtin_amdgcn_alignbit(m8, compact_diff(e[t], CF.qi, CF.bias), 31u);
                         mask |= m8 << u;
                     }
                 } else {
@@ -997,7 +1002,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     // the reorder wrote the compact copy of these positions
     const float qs = compact_scale(G);
     CompactFilter cf{};
-    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_T(c.s_max, qs)};
+    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
     for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                          cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                          [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
@@ -1196,7 +1201,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         float sum = 0.f;
         const float qs = compact_scale(G);
         CompactFilter cf{};
-        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_T(c.s_max, qs)};
+        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
         // recorded for the pressure pass of this correction (same positions and cells)
         for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
