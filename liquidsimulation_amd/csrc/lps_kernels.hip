@@ -232,16 +232,23 @@ __device__ __forceinline__ uint32_t compact_diff(const uint2& e, const uint2& qi
     return (uint32_t)__builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, bias, true), true);
 }
 
+// Kept candidates visited per trip of pass 2 (their loads issue together),
+// and candidates per trip of the float filter.  Two and four keep every
+// stencil kernel at or below 64 VGPRs, i.e. eight waves per SIMD; the rare
+// paths (the float filter of a far particle or a replayed row's later
+// windows, and the checked division) otherwise set the register count of
+// the whole kernel.
+constexpr int VB = 2;
 template <class Acc, class Pre, class F>
 __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px,
                                              float py, float pz, float s_max, Acc& accept_extra, Pre& pre, F& fn) {
     while (mask) {
-        uint32_t kk[4];
+        uint32_t kk[VB];
         uint32_t cnt = 1u;
         kk[0] = (uint32_t)__ffs((int)mask) - 1u;
         mask &= mask - 1u;
 #pragma unroll
-        for (int t = 1; t < 4; t++) {
+        for (int t = 1; t < VB; t++) {
             kk[t] = kk[0];
             if (mask) {
                 kk[t] = (uint32_t)__ffs((int)mask) - 1u;
@@ -249,15 +256,15 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
                 cnt++;
             }
         }
-        float4 q[4];
-        decltype(pre(0u)) pv[4];
+        float4 q[VB];
+        decltype(pre(0u)) pv[VB];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < VB; t++) {
             q[t] = *reinterpret_cast<const float4*>(xb + ((w0 + kk[t]) << 4));
             pv[t] = pre(w0 + kk[t]);
         }
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < VB; t++) {
             // slots past cnt repeat candidate kk[0]; one predicate keeps the
             // exec-mask bookkeeping to a single region per candidate
             const uint32_t j = w0 + kk[t];
@@ -318,14 +325,14 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
                         mask |= m8 << u;
                     }
                 } else {
-                    for (uint32_t u = 0; u < n; u += 8u) {
+                    for (uint32_t u = 0; u < n; u += 4u) {
                         const char* p8 = xb + ((w0 + u) << 4);
-                        float4 q[8];
+                        float4 q[4];
 #pragma unroll
-                        for (int t = 0; t < 8; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
+                        for (int t = 0; t < 4; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
                         uint32_t m8 = 0u;
 #pragma unroll
-                        for (int t = 0; t < 8; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
+                        for (int t = 0; t < 4; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
                         mask |= m8 << u;
                     }
                 }
