@@ -232,14 +232,15 @@ __device__ __forceinline__ uint32_t compact_diff(const uint2& e, const uint2& qi
     return (uint32_t)__builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, bias, true), true);
 }
 
-// Kept candidates visited per trip of pass 2 (their loads issue together),
-// and candidates per trip of the float filter.  Two and four keep every
-// stencil kernel at or below 64 VGPRs, i.e. eight waves per SIMD; the rare
-// paths (the float filter of a far particle or a replayed row's later
-// windows, and the checked division) otherwise set the register count of
-// the whole kernel.
-constexpr int VB = 2;
-template <class Acc, class Pre, class F>
+// Pass 2 visits VB kept candidates per trip (their loads issue together);
+// the float filter tests four candidates per trip.  VB = 2 keeps every
+// stencil kernel at or below 64 VGPRs, i.e. eight waves per SIMD: the rare
+// paths (the float filter of a far particle or of a replayed row's later
+// windows, and the checked division) otherwise set the register count of the
+// whole kernel.  A row holds about four kept candidates, so VB = 3 or 4
+// mostly fills slots with repeats, and VB = 1 leaves one load in flight;
+// both measured slower at 524k particles.
+template <int VB, class Acc, class Pre, class F>
 __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px,
                                              float py, float pz, float s_max, Acc& accept_extra, Pre& pre, F& fn) {
     while (mask) {
@@ -275,7 +276,7 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
     }
 }
 
-template <class Acc, class Pre, class F>
+template <int VB, class Acc, class Pre, class F>
 __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const uint32_t* __restrict__ lb, float ax,
                                                      float ay, float az, const float4* __restrict__ xs, float px,
                                                      float py, float pz, float s_max, float s_relax,
@@ -305,7 +306,7 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
             const uint32_t e = lb[base + (uint32_t)x1 + 1u];
             uint32_t w0 = b;
             if (replay) {
-                visit_window(xb, b, row[i], px, py, pz, s_max, accept_extra, pre, fn);
+                visit_window<VB>(xb, b, row[i], px, py, pz, s_max, accept_extra, pre, fn);
                 w0 += 32u;
             }
             uint32_t first = 0u;
@@ -338,7 +339,7 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
                 }
                 mask &= n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u;
                 if (w0 == b) first = mask;
-                visit_window(xb, w0, mask, px, py, pz, s_max, accept_extra, pre, fn);
+                visit_window<VB>(xb, w0, mask, px, py, pz, s_max, accept_extra, pre, fn);
             }
             if (record) row[i] = first;
         }
@@ -1010,7 +1011,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     const float qs = compact_scale(G);
     CompactFilter cf{};
     if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
+    for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                          cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                          [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
     const float rho = c.mass * sum;
@@ -1056,7 +1057,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
     float nb = 0.f, dv = 0.f;  // warm-start features: neighbour count, velocity divergence
     // the density pass recorded this stencil's masks at the same positions
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+    for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                          MC_REPLAY, CompactFilter{}, [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
                          [&](uint32_t j, const float4& xj, float s2, const float4& vj, const float3&) {
         const uint32_t idx = table_index(s2, sT, c.inv_div);
@@ -1154,7 +1155,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const float tf = q[i];
     const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors_masked(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+    for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                          MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
                          [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
         const float d = sqrt_rn(s2);
@@ -1210,7 +1211,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
         CompactFilter cf{};
         if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
         // recorded for the pressure pass of this correction (same positions and cells)
-        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+        for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                              [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
         const float rp = c.mass * sum;
@@ -1402,7 +1403,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     auto pressure = [&](auto nocheck) {
         // the predicted-density pass of this correction recorded the masks
-        for_neighbors_masked(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+        for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; },
                              [&](uint32_t j) { return q[j]; },
                              [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
