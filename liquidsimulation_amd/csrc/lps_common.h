@@ -11,7 +11,7 @@
 #include "lps.h"
 
 #define LPS_BLOCK 256           // threads per block for particle kernels (4 wave64)
-#define LPS_RADIX_ITEMS 8       // keys per thread per radix tile
+#define LPS_RADIX_ITEMS 4       // keys per thread per radix tile
 #define LPS_RADIX_TILE (LPS_BLOCK * LPS_RADIX_ITEMS)
 #define LPS_SCAN_ITEMS 16       // cells per thread in the cell-table scan
 #define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)

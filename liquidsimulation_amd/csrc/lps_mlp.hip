@@ -140,12 +140,17 @@ __device__ __forceinline__ void forward(const MlpSmem& S, const float* __restric
     F.y = part + S.b3;
 }
 
-// The inference weights in their BF16 LDS layout, packed once per change of
-// the FP32 master copy (k_mlp_pack) so that the seed kernel copies them
-// instead of converting them in every block.
+// The inference weights in BF16, packed once per change of the FP32 master
+// copy (k_mlp_pack) so that the seed kernel copies them instead of converting
+// them in every block.  The weight matrices are stored as MFMA A-operand
+// fragments: fragment t holds, for lane l, the four values the lane passes
+// to v_mfma_f32_16x16x16_bf16 (row 16 m + l%16, columns 16 k + 4 (l/16)..+3),
+// at [t][l][4].  A wave's fragment read is then 512 contiguous bytes, free of
+// LDS bank conflicts (a [row][column] layout puts the 16 rows a lane group
+// reads 128 bytes apart, all in one bank).
 struct alignas(16) MlpPacked {
-    unsigned short w1[HID * IN];    // [neuron][feature]
-    unsigned short w2[HID * HID];   // [out][in]
+    unsigned short w1[HID * IN];    // fragments (m), m < 4: W1[16 m + l%16][4 (l/16) + r]
+    unsigned short w2[HID * HID];   // fragments (4 m + k): W2[16 m + l%16][16 k + 4 (l/16) + r]
     float b1[HID], b2[HID], w3[HID];
     float mean[IN], istd[IN];
     float b3;
@@ -158,7 +163,7 @@ static_assert(sizeof(MlpPacked) % 16 == 0, "packed weights are copied as uint4")
 // keeping the pre-activations and hidden tiles the gradient needs.
 // xin holds features 4*(l/16)..+3 of the lane's sample.
 __device__ __forceinline__ float infer(const MlpPacked& S, const float4& xin, bool valid) {
-    const uint32_t l = threadIdx.x & 63u, g = l >> 4, r16 = l & 15u;
+    const uint32_t l = threadIdx.x & 63u, g = l >> 4;
     const float xv[4] = {xin.x, xin.y, xin.z, xin.w};
     float v[4];
     for (int r = 0; r < 4; r++) {
@@ -168,7 +173,7 @@ __device__ __forceinline__ float infer(const MlpPacked& S, const float4& xin, bo
     const bf16x4 x = f2bf4(v[0], v[1], v[2], v[3]);
     bf16x4 h1[4];
     for (int mt = 0; mt < 4; mt++) {
-        const f32x4 y = mfma(load4(&S.w1[(mt * 16 + r16) * IN + 4 * g]), x, f32x4{0.f, 0.f, 0.f, 0.f});
+        const f32x4 y = mfma(load4(&S.w1[(mt * 64 + l) * 4]), x, f32x4{0.f, 0.f, 0.f, 0.f});
         const float* b1 = &S.b1[mt * 16 + 4 * g];
         h1[mt] = f2bf4(fmaxf(y[0] + b1[0], 0.f), fmaxf(y[1] + b1[1], 0.f), fmaxf(y[2] + b1[2], 0.f),
                        fmaxf(y[3] + b1[3], 0.f));
@@ -176,7 +181,7 @@ __device__ __forceinline__ float infer(const MlpPacked& S, const float4& xin, bo
     float part = 0.f;
     for (int mt = 0; mt < 4; mt++) {
         f32x4 acc{0.f, 0.f, 0.f, 0.f};
-        for (int kt = 0; kt < 4; kt++) acc = mfma(load4(&S.w2[(mt * 16 + r16) * HID + kt * 16 + 4 * g]), h1[kt], acc);
+        for (int kt = 0; kt < 4; kt++) acc = mfma(load4(&S.w2[((mt * 4 + kt) * 64 + l) * 4]), h1[kt], acc);
         const float* b2 = &S.b2[mt * 16 + 4 * g];
         const float* w3 = &S.w3[mt * 16 + 4 * g];
         const uint32_t lo = f2bf2(fmaxf(acc[0] + b2[0], 0.f), fmaxf(acc[1] + b2[1], 0.f));
@@ -193,8 +198,16 @@ __device__ __forceinline__ float infer(const MlpPacked& S, const float4& xin, bo
 
 __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_pack(const float* __restrict__ params,
                                                         const float* __restrict__ fscale, MlpPacked* __restrict__ out) {
-    for (uint32_t k = threadIdx.x; k < HID * IN; k += blockDim.x) out->w1[k] = f2bf(params[OFF_W1 + k]);
-    for (uint32_t k = threadIdx.x; k < HID * HID; k += blockDim.x) out->w2[k] = f2bf(params[OFF_W2 + k]);
+    // element e of the fragment array: r = e % 4, lane l = (e / 4) % 64,
+    // fragment t = e / 256
+    for (uint32_t e = threadIdx.x; e < HID * IN; e += blockDim.x) {
+        const uint32_t r = e & 3u, l = (e >> 2) & 63u, m = e >> 8;
+        out->w1[e] = f2bf(params[OFF_W1 + (16u * m + (l & 15u)) * IN + 4u * (l >> 4) + r]);
+    }
+    for (uint32_t e = threadIdx.x; e < HID * HID; e += blockDim.x) {
+        const uint32_t r = e & 3u, l = (e >> 2) & 63u, t = e >> 8, m = t >> 2, k = t & 3u;
+        out->w2[e] = f2bf(params[OFF_W2 + (16u * m + (l & 15u)) * HID + 16u * k + 4u * (l >> 4) + r]);
+    }
     for (uint32_t k = threadIdx.x; k < HID; k += blockDim.x) {
         out->b1[k] = params[OFF_B1 + k];
         out->b2[k] = params[OFF_B2 + k];
@@ -240,7 +253,10 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(const MlpPacked* __re
 // makes it independent of particle size and time step), clamped at zero when
 // negative pressures are disabled, non-finite values replaced by 0;
 // q_i = p_i / rho*_i^2 for the pressure-force pass.
-__global__ __launch_bounds__(LPS_BLOCK) void k_mlp_seed(const MlpPacked* __restrict__ packed, float tscale,
+// Four waves per SIMD: the compiler keeps the layer weights and biases in
+// registers across the four sample groups (128 VGPRs, no spills; five waves
+// would spill).
+__global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_seed(const MlpPacked* __restrict__ packed, float tscale,
                                                         const float* __restrict__ feat, uint32_t fs, uint32_t n,
                                                         float* __restrict__ press, float* __restrict__ q,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
