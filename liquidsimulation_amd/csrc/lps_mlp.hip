@@ -223,10 +223,21 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_pack(const float* __restrict_
     }
 }
 
+// Copy of the packed weights into LDS by a block of LPS_BLOCK threads: every
+// load is issued before the first store, so the copy waits for one memory
+// round trip rather than one per row of the block.
 __device__ __forceinline__ void load_packed(MlpPacked& S, const MlpPacked* __restrict__ packed) {
+    constexpr uint32_t N = sizeof(MlpPacked) / 16u;
+    static_assert(N > 2u * LPS_BLOCK && N <= 3u * LPS_BLOCK, "three rows of uint4 per thread");
     const uint4* src = reinterpret_cast<const uint4*>(packed);
     uint4* dst = reinterpret_cast<uint4*>(&S);
-    for (uint32_t k = threadIdx.x; k < sizeof(MlpPacked) / 16u; k += blockDim.x) dst[k] = src[k];
+    const uint32_t t = threadIdx.x;
+    const bool third = t + 2u * LPS_BLOCK < N;
+    const uint4 a = src[t], b = src[t + LPS_BLOCK];
+    const uint4 c = third ? src[t + 2u * LPS_BLOCK] : make_uint4(0u, 0u, 0u, 0u);
+    dst[t] = a;
+    dst[t + LPS_BLOCK] = b;
+    if (third) dst[t + 2u * LPS_BLOCK] = c;
 }
 
 // Network output (in units of the target scale) of n rows of x.
@@ -264,31 +275,34 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
                                                         lps_dev::Scenes sc, LoopRule rule, float rest_density) {
     if (g->status != 0u) return;
     __shared__ MlpPacked S;
-    load_packed(S, packed);
-    __syncthreads();
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63u;
     // one block per 256-slot chunk, which is one particle block and so belongs
     // to one scene; chunks of scenes that stopped at the first correction are
-    // skipped.  The wave's four groups of 16 samples load their features first.
+    // skipped.  The wave's four groups of 16 samples load their features
+    // first, so that their latency overlaps the copy of the weights.
     const uint32_t bb = blockIdx.x * LPS_BLOCK;
+    // features are feature-major (feature f of slot p at f * fs + p)
+    float4 xin[4];
+    for (uint32_t grp = 0; grp < 4; grp++) {
+        const uint32_t p = min(bb + w * 64u + grp * 16u + (l & 15u), n - 1u);
+        const float* fp = feat + (size_t)(4u * (l >> 4)) * fs + p;
+        xin[grp] = make_float4(fp[0], fp[fs], fp[2 * (size_t)fs], fp[3 * (size_t)fs]);
+    }
+    load_packed(S, packed);
     {
         // the seed replaces the first correction's pressure unless the scene's
         // loop stops there (the loop rule at k = 1, which the pressure pass
         // applies: a non-finite error, the tolerance met with min <= 1, or max
-        // <= 1; the stall rule cannot stop a loop at its first correction)
+        // <= 1; the stall rule cannot stop a loop at its first correction).
+        // The test is uniform over the block, so the whole block leaves
+        // before the barrier or none of it does.
         const uint32_t s = sc.block_scene[blockIdx.x];
         const uint32_t eb = sc.grid[s].err_bits[0];
         if (eb >= 0x7F800000u || (1u >= rule.min_iter && __uint_as_float(eb) / rest_density < rule.tol) ||
             1u >= rule.max_iter)
             return;
         const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
-        // features are feature-major (feature f of slot p at f * fs + p)
-        float4 xin[4];
-        for (uint32_t grp = 0; grp < 4; grp++) {
-            const uint32_t p = min(bb + w * 64u + grp * 16u + (l & 15u), n - 1u);
-            const float* fp = feat + (size_t)(4u * (l >> 4)) * fs + p;
-            xin[grp] = make_float4(fp[0], fp[fs], fp[2 * (size_t)fs], fp[3 * (size_t)fs]);
-        }
+        __syncthreads();
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
             const float y = infer(S, xin[grp], p < end);
