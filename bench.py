@@ -326,7 +326,7 @@ def main():
                     help="skip the event-timed rerun that gives per-kernel averages")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-warm", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--mode", choices=("both", "plain"), default="both")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy five times (scales the HBM byte counters)")
