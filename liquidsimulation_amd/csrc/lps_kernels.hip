@@ -191,10 +191,10 @@ enum : int { MC_LIVE = 0, MC_RECORD = 1, MC_REPLAY = 2 };
 //   with floats instead; its neighbours then all lie within 2^22.
 // Far predictions (beyond 2^22) store a saturated value that can only add
 // candidates.  The dot products start from the bias -(T + 1), T the squared
-// radius in units, so a kept candidate is one whose saturated result is
-// negative: a kept pair's s_q - (T + 1) lies in [-(T + 1), -1], and larger
-// sums (up to 3 * 2^30) saturate at 2^31 - 1.  alignbit shifts each sign bit
-// into the mask, one instruction per candidate.
+// radius in units, so a candidate is kept exactly when the saturated result
+// is negative: s_q <= T gives s_q - (T + 1) in [-(T + 1), -1], and s_q > T a
+// positive result (sums up to 3 * 2^30 saturate at 2^31 - 1).  alignbit
+// shifts each sign bit into the mask, one instruction per candidate.
 struct CompactFilter {
     const uint2* cq;  // nullptr: float filter
     uint2 qi;         // this particle's entry
