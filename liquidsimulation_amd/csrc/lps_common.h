@@ -13,7 +13,7 @@
 #define LPS_BLOCK 256           // threads per block for particle kernels (4 wave64)
 #define LPS_RADIX_ITEMS 4       // keys per thread per radix tile
 #define LPS_RADIX_TILE (LPS_BLOCK * LPS_RADIX_ITEMS)
-#define LPS_SCAN_ITEMS 16       // cells per thread in the cell-table scan
+#define LPS_SCAN_ITEMS 4       // cells per thread in the cell-table scan
 #define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)
 #define LPS_MAX_ITER_SLOTS 64   // per-iteration error slots (iterations beyond reuse the last)
 #define LPS_MAX_COLLIDERS 1024    // per context (all scenes)
