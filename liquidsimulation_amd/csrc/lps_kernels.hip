@@ -609,10 +609,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ p
 // Grid parameters of each scene (SPHSolver.cpp:1031-1040), one block per
 // scene over the partial boxes of its particle blocks.  min starts at FLT_MAX
 // and max at FLT_MIN as in the reference (SPHSolver.cpp:1013-1014).
-__global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, const uint32_t* __restrict__ part,
-                                                          float h) {
+__device__ __forceinline__ void scene_grid_block(const lps_dev::Scenes& S, const uint32_t* __restrict__ part,
+                                                 float h, uint32_t sidx) {
     __shared__ uint32_t s_r[7][LPS_BLOCK / 64];
-    const uint32_t sidx = blockIdx.x;
     const SceneInfo si = S.info[sidx];
     const uint32_t nparts = si.padded / LPS_BLOCK;
     uint32_t r[7];
@@ -636,7 +635,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, con
     if (threadIdx.x != 0) return;
     for (int a = 0; a < 7; a++)
         for (uint32_t k = 1; k < LPS_BLOCK / 64; k++) r[a] = (a < 3) ? min(r[a], s_r[a][k]) : max(r[a], s_r[a][k]);
-    SceneGrid* sg = S.grid + sidx;
+    SceneGrid* sg = S.grid + sidx;  // written by thread 0 only
     uint32_t bad = r[6] ? 1u : 0u;
     float mn[3], mx[3];
     const float gs = h * 1.f;
@@ -661,11 +660,16 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, con
     sg->bad = bad;
 }
 
+__global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, const uint32_t* __restrict__ part,
+                                                          float h) {
+    scene_grid_block(S, part, h, blockIdx.x);
+}
+
 // Cell numbering across scenes (scene s owns cells [cell_base, cell_base +
 // cells] including its empty-slot cell), capacity check, and the per-step
-// reset of the loop state.
-__global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::Scenes S, float h,
-                                                           uint32_t cell_cap) {
+// reset of the loop state.  Thread 0 reads every scene's grid.
+__device__ __forceinline__ void grid_finish_block(DevGrid* g, const lps_dev::Scenes& S, float h,
+                                                  uint32_t cell_cap) {
     const uint32_t t = threadIdx.x;
     for (uint32_t b = t; b < S.nblocks; b += LPS_BLOCK) S.act_list[b] = b;
     for (uint32_t sidx = t; sidx < S.count; sidx += LPS_BLOCK) {
@@ -703,6 +707,21 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::
     g->act_count[1] = 0u;
     g->list_of[1] = 0u;
     if (!status) g->steps_ok += 1u;
+}
+
+__global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::Scenes S, float h,
+                                                           uint32_t cell_cap) {
+    grid_finish_block(g, S, h, cell_cap);
+}
+
+// Both of the above for a single scene, in one block: thread 0 writes the
+// scene's grid and then reads it (one launch less per step).
+__global__ __launch_bounds__(LPS_BLOCK) void k_grid_one(DevGrid* g, lps_dev::Scenes S,
+                                                        const uint32_t* __restrict__ part, float h,
+                                                        uint32_t cell_cap) {
+    scene_grid_block(S, part, h, 0u);
+    __syncthreads();
+    grid_finish_block(g, S, h, cell_cap);
 }
 
 // calcGridIndices (SPHKernels.cl:66-82) within the scene's grid, offset by the
@@ -1519,8 +1538,12 @@ void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc,
 uint32_t bbox_parts(uint32_t n) { return cdiv(n, LPS_BLOCK); }
 
 void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap) {
-    k_scene_grid<<<sc.count, LPS_BLOCK, 0, s>>>(sc, part, h);
-    k_grid_finish<<<1, LPS_BLOCK, 0, s>>>(g, sc, h, cell_cap);
+    if (sc.count == 1u) {
+        k_grid_one<<<1, LPS_BLOCK, 0, s>>>(g, sc, part, h, cell_cap);
+    } else {
+        k_scene_grid<<<sc.count, LPS_BLOCK, 0, s>>>(sc, part, h);
+        k_grid_finish<<<1, LPS_BLOCK, 0, s>>>(g, sc, h, cell_cap);
+    }
 }
 
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
