@@ -399,7 +399,7 @@ void enqueue_grid(lps_ctx* c) {
     }
     {
         Timed t(c, "keys");
-        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, scenes(c), c->keysA, c->valsA);
+        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, scenes(c), c->keysA, c->valsA, c->block_max);
     }
     for (uint32_t p = 0; p < c->passes; p++) {
         Timed t(c, "radix_pass");
@@ -420,6 +420,7 @@ void enqueue_grid(lps_ctx* c) {
         r.pos_out = c->pos[b]; r.vel_out = c->vel[b]; r.vh_out = c->vh[b]; r.first_out = c->first[b];
         r.id_out = c->id[b]; r.pprev_out = c->pprev[b];
         r.run_end = c->run_end;
+        r.tile_max = c->block_max;
         r.skeys = c->skeys;
         r.perm = c->perm;
         r.sc = scenes(c);

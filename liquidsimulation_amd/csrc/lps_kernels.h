@@ -52,6 +52,7 @@ struct ReorderArgs {
     uint32_t* id_out;
     float* pprev_out;
     uint32_t* run_end;
+    uint32_t* tile_max;  // per-tile maxima of the run ends (cell-table scan tiles)
     uint32_t* skeys;
     uint32_t* perm;
     Scenes sc;            // slot -> scene (for the compact copy's grid)
@@ -81,7 +82,8 @@ void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc,
 uint32_t bbox_parts(uint32_t n);
 void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap);
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
-                 uint32_t* vals);
+                 uint32_t* vals,
+                 uint32_t* tile_max);
 uint32_t radix_tiles(uint32_t n);
 void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist, uint32_t* digit_total);

@@ -729,9 +729,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_one(DevGrid* g, lps_dev::Sce
 // after its particles.
 __global__ __launch_bounds__(LPS_BLOCK) void k_keys(const float4* __restrict__ pos, uint32_t n, const DevGrid* g,
                                                     lps_dev::Scenes S, uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals) {
+                                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ tile_max) {
     if (step_dead(g)) return;
     const uint32_t i = blockIdx.x * LPS_BLOCK + threadIdx.x;
+    // the cell table's per-tile maxima of the run ends start at zero; the
+    // reorder writes those of the non-empty tiles
+    const uint32_t ntiles = (g->total_cells + LPS_SCAN_TILE) / LPS_SCAN_TILE;
+    for (uint32_t b = i; b < ntiles; b += gridDim.x * LPS_BLOCK) tile_max[b] = 0u;
     if (i >= n) return;
     const BlockScene B = block_scene(S, blockIdx.x);
     const SceneGrid* sg = S.grid + B.s;
@@ -887,7 +891,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
     float4* __restrict__ vel_out, float4* __restrict__ vh_out, uint8_t* __restrict__ first_out,
     uint32_t* __restrict__ id_out, float* __restrict__ pprev_out, uint32_t* __restrict__ run_end,
     uint32_t* __restrict__ skeys_out, uint32_t* __restrict__ perm_out, lps_dev::Scenes S, uint2* __restrict__ cq,
-    uint32_t nlogical) {
+    uint32_t* __restrict__ tile_max, uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
@@ -902,7 +906,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
         const uint32_t k = keys[i];
         skeys_out[i] = k;
         perm_out[i] = j;
-        if (i == n - 1u || keys[i + 1u] != k) run_end[k] = i + 1u;
+        if (i == n - 1u || keys[i + 1u] != k) {
+            run_end[k] = i + 1u;
+            // the last run of a cell-table tile holds the tile's largest run
+            // end (keys are sorted): one plain store per non-empty tile
+            if (i == n - 1u || keys[i + 1u] / LPS_SCAN_TILE != k / LPS_SCAN_TILE) tile_max[k / LPS_SCAN_TILE] = i + 1u;
+        }
     }
     const float4 p = pos_in[j];
     pos_out[i] = p;
@@ -922,29 +931,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_reorder(
 
 // Cell table = exclusive max-scan of the run ends: LB[c] = #keys < c, the
 // cellList of buildCellList (SPHKernels.cl:148-186), with LB[cells] = n.  The
-// run-end marks are cleared as they are consumed.
-__global__ __launch_bounds__(LPS_BLOCK) void k_cell_reduce(const uint32_t* __restrict__ run_end, const DevGrid* g,
-                                                           uint32_t* __restrict__ block_max) {
-    if (step_dead(g)) return;
-    const uint32_t total = g->total_cells + 1u;
-    const uint32_t base = blockIdx.x * LPS_SCAN_TILE;
-    if (base >= total) return;
-    uint32_t m = 0;
-    for (uint32_t k = 0; k < LPS_SCAN_ITEMS; k++) {
-        const uint32_t c = base + k * LPS_BLOCK + threadIdx.x;
-        if (c < total) m = max(m, run_end[c]);
-    }
-    m = wave_max_u32(m);
-    __shared__ uint32_t s[LPS_BLOCK / 64];
-    if (lane_id() == 0) s[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t r = 0;
-        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) r = max(r, s[k]);
-        block_max[blockIdx.x] = r;
-    }
-}
-
+// run-end marks are cleared as they are consumed.  block_max holds each
+// tile's largest run end (0 for an empty tile), written by the reorder after
+// k_keys cleared it.
 __global__ __launch_bounds__(LPS_BLOCK) void k_cell_scan(uint32_t* __restrict__ run_end, const DevGrid* g,
                                                          const uint32_t* __restrict__ block_max,
                                                          uint32_t* __restrict__ cell_lb) {
@@ -1547,8 +1536,8 @@ void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32
 }
 
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
-                 uint32_t* vals) {
-    k_keys<<<cdiv(n, LPS_BLOCK), LPS_BLOCK, 0, s>>>(pos, n, g, sc, keys, vals);
+                 uint32_t* vals, uint32_t* tile_max) {
+    k_keys<<<cdiv(n, LPS_BLOCK), LPS_BLOCK, 0, s>>>(pos, n, g, sc, keys, vals, tile_max);
 }
 
 uint32_t radix_tiles(uint32_t n) { return cdiv(n, LPS_RADIX_TILE); }
@@ -1566,7 +1555,7 @@ void launch_reorder(hipStream_t s, const ReorderArgs& a) {
     k_reorder<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.keysA, a.valsA, a.keysB, a.valsB, a.n, a.g, a.pos_in, a.vel_in,
                                                  a.vh_in, a.first_in, a.id_in, a.pprev_in, a.pos_out, a.vel_out,
                                                  a.vh_out, a.first_out, a.id_out, a.pprev_out, a.run_end,
-                                                 a.skeys, a.perm, a.sc, a.cq, nl);
+                                                 a.skeys, a.perm, a.sc, a.cq, a.tile_max, nl);
 }
 
 uint32_t cell_blocks(uint32_t cell_cap) { return cdiv(cell_cap + 1u, LPS_SCAN_TILE); }
@@ -1574,7 +1563,6 @@ uint32_t cell_blocks(uint32_t cell_cap) { return cdiv(cell_cap + 1u, LPS_SCAN_TI
 void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint32_t* block_max, uint32_t* cell_lb,
                        uint32_t cell_cap) {
     const uint32_t nb = cell_blocks(cell_cap);
-    k_cell_reduce<<<nb, LPS_BLOCK, 0, s>>>(run_end, g, block_max);
     k_cell_scan<<<nb, LPS_BLOCK, 0, s>>>(run_end, g, block_max, cell_lb);
 }
 
