@@ -74,8 +74,10 @@ typedef struct lps_params {
     float max_density_error_ratio;   /* setMaxDensityErrorRatio: stop when max|e|/rho0 < this */
     int32_t warm_start;              /* lps_warm_start */
     int32_t stall_iterations;        /* also stop once the error has not improved on its best value
-                                        for this many corrections (0: off); such steps are reported
-                                        as not converged */
+                                        (from the second correction on) for this many corrections
+                                        (0: off); such a step applies the pressure force whose
+                                        predictions had that best error, reports that error and is
+                                        reported as not converged */
 } lps_params;
 
 /* Static colliders (StaticCollisionBox / StaticCollisionSphere, include/Collision/),
@@ -86,7 +88,9 @@ typedef struct lps_plane { float normal[3]; float offset; } lps_plane; /* fluid 
 
 typedef struct lps_step_stats {
     int32_t iterations;              /* PCISPH corrections run this step (0 for SPH) */
-    float max_density_error_ratio;   /* max_i |e_i| / rho0 after the last correction */
+    float max_density_error_ratio;   /* max_i |e_i| / rho0 of the applied pressure force's
+                                        predictions (the last correction's; the best one's after a
+                                        stall stop) */
     int32_t warm_start_used;         /* 1 if the learned seed was applied */
     int32_t status;                  /* lps_status of this step */
     int32_t converged;               /* 1: tolerance met; 0: stopped at the cap or by the stall rule */
@@ -221,10 +225,22 @@ int lps_get_fields(lps_ctx* ctx, float* rho, float* pressure, float* force /* [3
  * do not depend on particle size or time step.
  * Parameters (fp32 master copy): W1[64][16], b1[64], W2[64][64], b2[64],
  * W3[64], b3.  Inputs are normalised as (f - mean) * inv_std; the seed is the
- * output times target_scale times P.  With warm_start = LPS_WARM_LEARNED and weights
+ * output times target_scale times P (0 when not finite).  With negative
+ * pressures disabled the loop never lowers a pressure, so the first
+ * correction's pressure becomes max(p1, min(seed, LPS_WARM_KAPPA * p_prev))
+ * (p1: the zero start's own first-correction pressure; p_prev: the
+ * particle's applied pressure of the previous step): an over-prediction
+ * shrinks by LPS_WARM_KAPPA per step instead of feeding itself through
+ * feature 7.  With negative pressures enabled it becomes the seed.
+ * With warm_start = LPS_WARM_LEARNED and weights
  * loaded the seed is used unless the running mean of corrections per step
  * with it exceeds the one without it (every 32nd step runs the other mode to
  * keep both current); a seeded step that turns non-finite is re-run unseeded. */
+/* Seed envelope factor: with negative pressures disabled the first
+ * correction's pressure is max(p1, min(seed, LPS_WARM_KAPPA * p_prev)), p1
+ * the zero start's own first-correction pressure and p_prev the particle's
+ * applied pressure of the previous step. */
+#define LPS_WARM_KAPPA 0.9f
 #define LPS_MLP_IN 16
 #define LPS_MLP_HIDDEN 64
 #define LPS_MLP_PARAMS (LPS_MLP_IN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + 1)
@@ -233,8 +249,9 @@ int lps_mlp_set_weights(lps_ctx* ctx, const float* params /* LPS_MLP_PARAMS, fp3
 int lps_mlp_get_weights(lps_ctx* ctx, float* params, float* feature_scale, float* target_scale);
 /* Compute the features on every PCISPH step (for lps_dump_pairs) when on. */
 int lps_mlp_capture(lps_ctx* ctx, int32_t on);
-/* Features, final pressure and (if the step was seeded) the seed of the last
- * PCISPH step, sorted order, pressures in Pa.  Any pointer may be NULL. */
+/* Features, applied pressure and (if the step was seeded) the network's seed
+ * (before the envelope) of the last PCISPH step, sorted order, pressures in
+ * Pa.  Any pointer may be NULL. */
 int lps_dump_pairs(lps_ctx* ctx, float* features /* [n*16] */, float* target /* [n] */,
                    float* seed /* [n] */, uint32_t n);
 /* Device-resident training set: while capacity > 0 and stride > 0, every

@@ -21,6 +21,8 @@ METHOD_PCISPH = 1
 WARM_NONE = 0
 WARM_LEARNED = 1
 WARM_LEARNED_ALWAYS = 2
+# seed envelope factor of the learned warm start (LPS_WARM_KAPPA, a float)
+WARM_KAPPA = 0.9
 
 STATUS_NAMES = {
     0: "LPS_OK",

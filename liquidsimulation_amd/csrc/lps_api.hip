@@ -321,10 +321,12 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if ((st = dalloc(&c->bbox_part, (size_t)LPS_BBOX_PART * lps_dev::bbox_parts(n)))) return st;
     if ((st = dalloc(&c->posrho, (size_t)n + LPS_STENCIL_PAD))) return st;
     if ((st = dalloc(&c->force, n))) return st;
-    if ((st = dalloc(&c->fp, n))) return st;
-    if ((st = dalloc(&c->press, n))) return st;
+    // pressure force, pressure and predicted density: three buffers each
+    // (a correction's own, the previous one's, the best one's)
+    if ((st = dalloc(&c->fp, 3 * n))) return st;
+    if ((st = dalloc(&c->press, 3 * n))) return st;
     if ((st = dalloc(&c->q, n))) return st;
-    if ((st = dalloc(&c->rho_pred, n))) return st;
+    if ((st = dalloc(&c->rho_pred, 3 * n))) return st;
     if ((st = dalloc(&c->mk_mask, (size_t)LPS_MC_ROWS * n))) return st;
     if ((st = dalloc(&c->mk_ok, n))) return st;
     if ((st = dalloc(&c->mk_cq, (size_t)n + LPS_STENCIL_PAD))) return st;
@@ -528,9 +530,9 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
         // the first correction's pressure update is replaced by the seed
         Timed t(c, "mlp_seed");
         lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->cap_n, c->n, c->press, c->q,
-                                 c->seed, c->rho_pred, delta * c->params.rest_density,
-                                 c->params.negative_pressure_factor == 0.f ? 1 : 0, c->grid, scenes(c), rule,
-                                 c->params.rest_density);
+                                 c->seed, c->rho_pred, c->pprev[c->cur], LPS_WARM_KAPPA,
+                                 delta * c->params.rest_density, c->params.negative_pressure_factor == 0.f ? 1 : 0,
+                                 c->grid, scenes(c), rule, c->params.rest_density);
     }
     {
         Timed t(c, "pci_pressure");
@@ -648,23 +650,24 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
         k_known = kw;
         if (kw >= kmax) break;
     }
+    {
+        Timed t(c, "integrate");
+        lps_dev::launch_integrate(s, a, c->fp, c->pprev[c->cur], c->press, c->bbox_part);
+    }
     if (feat_on && res.ok && c->coll_stride > 0 && c->coll_rows < c->coll_cap) {
-        // training rows: every stride-th particle's features and final pressure
+        // training rows: every stride-th particle's features and the
+        // pressure the step applied (copied to pprev by the integration)
         const uint32_t off = c->coll_offset % c->coll_stride;
         for (const SceneInfo& si : c->h_sinfo) {
             const uint32_t n = si.count;
             uint64_t count = n > off ? ((uint64_t)(n - off) + c->coll_stride - 1u) / c->coll_stride : 0u;
             count = std::min<uint64_t>(count, c->coll_cap - c->coll_rows);
-            lps_dev::launch_collect(s, c->feat + si.slot0, c->cap_n, c->press + si.slot0,
+            lps_dev::launch_collect(s, c->feat + si.slot0, c->cap_n, c->pprev[c->cur] + si.slot0,
                                     delta * c->params.rest_density, n, c->coll_stride, off, (uint32_t)count,
                                     c->coll_x, c->coll_t, c->coll_rows);
             c->coll_rows += count;
         }
         c->coll_offset = (off + 1u) % c->coll_stride;
-    }
-    {
-        Timed t(c, "integrate");
-        lps_dev::launch_integrate(s, a, c->fp, c->pprev[c->cur], c->press, c->bbox_part);
     }
     c->last_fp_valid = 1;
     c->feat_valid = feat_on;
@@ -720,6 +723,23 @@ void compact_slots(const lps_ctx* c, const std::vector<T>& full, T* out, size_t 
         std::memcpy(out + r * width, full.data() + (size_t)si.slot0 * width, sizeof(T) * si.count * width);
         r += si.count;
     }
+}
+
+// Copies the slots of every scene from the buffer (of three, `slots` apart)
+// that the scene's loop applied in the last step into `full` (a whole slot
+// array, for compact_slots).  PCISPH only; the caller has synchronised.
+template <class T>
+int gather_applied(lps_ctx* c, const T* dev3, std::vector<T>& full) {
+    const size_t S = c->h_sinfo.size();
+    HIPC(hipMemcpy(c->h_sgrid, c->d_sgrid, sizeof(SceneGrid) * S, hipMemcpyDeviceToHost));
+    full.resize(c->n);
+    for (size_t k = 0; k < S; k++) {
+        const SceneInfo& si = c->h_sinfo[k];
+        const uint32_t b = std::min<uint32_t>(c->h_sgrid[k].final_buf, 2u);
+        HIPC(hipMemcpy(full.data() + si.slot0, dev3 + (size_t)b * c->cap_n + si.slot0, sizeof(T) * si.count,
+                       hipMemcpyDeviceToHost));
+    }
+    return LPS_OK;
 }
 
 }  // namespace
@@ -1109,9 +1129,7 @@ int lps_get_scene_stats(lps_ctx* c, lps_step_stats* stats, uint32_t nscenes) {
         const uint32_t it = sg.stop_iter;
         o.iterations = (int32_t)it;
         o.max_density_error_ratio =
-            it ? __builtin_bit_cast(float, sg.err_bits[std::min(it - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u)]) /
-                     c->params.rest_density
-               : 0.f;
+            it ? __builtin_bit_cast(float, sg.final_bits) / c->params.rest_density : 0.f;
         o.warm_start_used = c->seed_valid ? 1 : 0;
         o.status = (int32_t)c->h_grid->status;
         o.converged = (int32_t)sg.converged;
@@ -1418,16 +1436,16 @@ int lps_get_fields(lps_ctx* c, float* rho, float* pressure, float* force, float*
     if (rho)
         for (uint32_t i = 0; i < n; i++) rho[i] = ra[i].w;
     if (pressure) {
-        HIPC(hipMemcpy(f.data(), c->press, sizeof(float) * ns, hipMemcpyDeviceToHost));
+        // the applied pressure, copied by the integration
+        HIPC(hipMemcpy(f.data(), c->pprev[c->cur], sizeof(float) * ns, hipMemcpyDeviceToHost));
         compact_slots(c, f, pressure, 1);
     }
     if (force) {
         HIPC(hipMemcpy(a.data(), c->force, sizeof(float4) * ns, hipMemcpyDeviceToHost));
         compact_slots(c, a, ra.data(), 1);
         if (c->last_fp_valid) {
-            b.resize(ns);
             rb.resize(n);
-            HIPC(hipMemcpy(b.data(), c->fp, sizeof(float4) * ns, hipMemcpyDeviceToHost));
+            if (int st = gather_applied(c, c->fp, b)) return st;
             compact_slots(c, b, rb.data(), 1);
         }
         for (uint32_t i = 0; i < n; i++) {
@@ -1443,7 +1461,11 @@ int lps_get_fields(lps_ctx* c, float* rho, float* pressure, float* force, float*
         }
     }
     if (rho_pred) {
-        HIPC(hipMemcpy(f.data(), c->rho_pred, sizeof(float) * ns, hipMemcpyDeviceToHost));
+        if (c->last_fp_valid) {
+            if (int st = gather_applied(c, c->rho_pred, f)) return st;
+        } else {
+            HIPC(hipMemcpy(f.data(), c->rho_pred, sizeof(float) * ns, hipMemcpyDeviceToHost));
+        }
         compact_slots(c, f, rho_pred, 1);
     }
     return LPS_OK;
@@ -1507,7 +1529,8 @@ int lps_dump_pairs(lps_ctx* c, float* features, float* target, float* seed, uint
         compact_slots(c, f, features, LPS_MLP_IN);
     }
     if (target) {
-        HIPC(hipMemcpy(f.data(), c->press, sizeof(float) * (size_t)ns, hipMemcpyDeviceToHost));
+        // the applied pressure, copied by the integration
+        HIPC(hipMemcpy(f.data(), c->pprev[c->cur], sizeof(float) * (size_t)ns, hipMemcpyDeviceToHost));
         compact_slots(c, f, target, 1);
     }
     if (seed) {

@@ -15,7 +15,7 @@
 #define LPS_RADIX_TILE (LPS_BLOCK * LPS_RADIX_ITEMS)
 #define LPS_SCAN_ITEMS 4       // cells per thread in the cell-table scan
 #define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)
-#define LPS_MAX_ITER_SLOTS 64   // per-iteration error slots (iterations beyond reuse the last)
+#define LPS_MAX_ITER_SLOTS 64   // per-correction error slots, used as a ring: correction k owns slot (k - 1) % 64
 #define LPS_MAX_COLLIDERS 1024    // per context (all scenes)
 #define LPS_FEAT 16              // warm-start features per particle (LPS_MLP_IN)
 
@@ -50,10 +50,22 @@ struct SceneGrid {
     uint32_t cell_base;
     uint32_t bad;            // 1: non-finite position, 2: extent too large for a grid
     uint32_t stop_iter;      // correction after which this scene's loop stopped (0: running)
-    uint32_t best_bits;      // best (smallest) max |e| of this step's corrections so far
+    uint32_t best_bits;      // smallest max |e| of this step's corrections from the second on
+    uint32_t best_k;         // the correction that had it (0: none yet)
+    uint32_t best_buf;       // pressure buffer of correction best_k - 1 (its predictions came from it)
     uint32_t stall;          // corrections since the best value last improved
     uint32_t converged;      // the tolerance was met
-    uint32_t err_bits[LPS_MAX_ITER_SLOTS];  // max |e_i| (float bits) per PCISPH correction
+    uint32_t final_bits;     // max |e_i| (float bits) the loop reports: the last correction's, or
+                             // the best one's after a stall exit
+    uint32_t final_buf;      // pressure buffer the step applies (read by the integration)
+    // Pressure, predicted density and pressure force of a correction live in
+    // one of three buffers: correction k writes buffer buf_of[k % 64], which
+    // the loop rule of correction k - 1 chose as neither that correction's nor
+    // the best one's, so the best correction's inputs survive until the stop.
+    uint8_t buf_of[LPS_MAX_ITER_SLOTS];
+    // max |e_i| (float bits) of correction k in slot (k - 1) % 64; the loop
+    // rule of correction k clears the slot of correction k + 1
+    uint32_t err_bits[LPS_MAX_ITER_SLOTS];
 };
 
 // Global state of the current step.

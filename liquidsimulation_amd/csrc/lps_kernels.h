@@ -94,6 +94,9 @@ void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint3
 void launch_density(hipStream_t s, const PhysArgs& a, float* press, float* q);
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat);
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
+// rho_pred and press: three buffers of a.mk.stride slots each; correction k
+// writes the scene's buffer buf_of[k % 64] (and reads the previous pressure
+// from buf_of[(k - 1) % 64]); q is a single buffer
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
                         float* q, uint32_t k, float* feat);
 
@@ -115,14 +118,17 @@ struct RowMap {
         return idx ? idx[s] : (uint32_t)(((start + s) * mul + add) % rows);
     }
 };
-// p_i = pscale * tscale * MLP(features_i) (non-negative when negative
-// pressures are disabled, 0 when not finite), q_i = p_i / rho*_i^2,
-// seed_i = p_i.  No-op once the step has stopped or failed.
+// y_i = pscale * tscale * MLP(features_i) (0 when not finite) and seed_i =
+// y_i; the first correction's pressure (buffer buf_of[1] of press, stride
+// fs) becomes y_i, or with negative pressures disabled (clamp_nonneg)
+// max(p1_i, min(y_i, kappa * pprev_i)), p1_i the pressure that correction
+// computed; q_i = p_i / rho*_i^2.  No-op once the step has stopped or failed.
 // feat: feature-major, feature f of slot i at f * fs + i
 // skips the scenes whose loop stops at the first correction (rule, rest_density)
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
-                     float* press, float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg,
-                     const DevGrid* g, const Scenes& sc, const LoopRule& rule, float rest_density);
+                     float* press, float* q, float* seed, const float* rho_pred, const float* pprev, float kappa,
+                     float pscale, int clamp_nonneg, const DevGrid* g, const Scenes& sc, const LoopRule& rule,
+                     float rest_density);
 // BF16 inference weights (k_mlp_seed's LDS layout) from the FP32 master copy
 size_t mlp_packed_bytes();
 void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed);
@@ -139,6 +145,9 @@ void launch_collect(hipStream_t s, const float* feat, uint32_t fs, const float* 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
                          const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
                          uint32_t* host_flag);
+// PCISPH (fp given): fp and press hold three buffers of a.mk.stride slots;
+// each scene reads the one its loop applied (SceneGrid::final_buf).  SPH:
+// press is the density pass's pressure.  pprev receives the applied pressure.
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,
                       uint32_t* bbox_part);
 // positions of the real particles, scenes back to back, to dst[r * stride + 0..2]

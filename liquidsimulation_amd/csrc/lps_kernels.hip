@@ -676,8 +676,13 @@ __device__ __forceinline__ void grid_finish_block(DevGrid* g, const lps_dev::Sce
         SceneGrid* sg = S.grid + sidx;
         sg->stop_iter = 0u;
         sg->best_bits = 0u;
+        sg->best_k = 0u;
+        sg->best_buf = 0u;
         sg->stall = 0u;
         sg->converged = 0u;
+        sg->final_bits = 0u;
+        sg->final_buf = 0u;
+        sg->buf_of[1] = 0u;
         for (int k = 0; k < LPS_MAX_ITER_SLOTS; k++) sg->err_bits[k] = 0u;
     }
     if (t != 0) return;
@@ -1194,7 +1199,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
     float* __restrict__ q, uint32_t k, float* __restrict__ feat, const float4* __restrict__ posrho,
     lps_dev::Scenes S, lps_dev::NbMasks mk, uint32_t nlogical) {
-    const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
+    const uint32_t slot = (k - 1u) % LPS_MAX_ITER_SLOTS;
     // a correction queued past the stop, or of a dead step, does nothing
     if (step_dead(g) || g->running == 0u) return;
     // only blocks of scenes whose loop is running are scheduled
@@ -1229,9 +1234,14 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
             dp *= c.npf;
             e *= c.npf;
         }
-        const float p = (k == 1u ? 0.f : press[i]) + dp;
-        rho_pred[i] = rp;
-        press[i] = p;
+        // this correction's buffer (chosen by the previous loop rule) and
+        // the previous correction's pressure
+        const SceneGrid* sg = S.grid + B.s;
+        const size_t ob = (size_t)sg->buf_of[k % LPS_MAX_ITER_SLOTS] * mk.stride;
+        const float pold = k == 1u ? 0.f : press[(size_t)sg->buf_of[(k - 1u) % LPS_MAX_ITER_SLOTS] * mk.stride + i];
+        const float p = pold + dp;
+        rho_pred[ob + i] = rp;
+        press[ob + i] = p;
         q[i] = p / (rp * rp);
         if (FEAT && k == 1u) {
             float* fr = feat + i;  // feature-major, as in k_forces
@@ -1286,7 +1296,7 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
         }
         return;
     }
-    const uint32_t slot = min(k - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u);
+    const uint32_t slot = (k - 1u) % LPS_MAX_ITER_SLOTS;
     const uint32_t L = g->list_of[k % LPS_MAX_ITER_SLOTS];
     __shared__ uint32_t s_run, s_err, s_conv, s_nan, s_nb;
     if (threadIdx.x == 0) {
@@ -1300,35 +1310,63 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
     for (uint32_t sidx = threadIdx.x; sidx < S.count; sidx += LPS_BLOCK) {
         SceneGrid* sg = S.grid + sidx;
         if (sg->stop_iter != 0u) {
-            atomicMax(&s_err, sg->err_bits[min(sg->stop_iter - 1u, (uint32_t)LPS_MAX_ITER_SLOTS - 1u)]);
+            atomicMax(&s_err, sg->final_bits);
             if (!sg->converged) atomicAnd(&s_conv, 0u);
             continue;
         }
         const uint32_t eb = sg->err_bits[slot];
+        // the density pass of correction k + 1 (after this kernel) starts
+        // from a cleared slot, also once the ring has wrapped
+        sg->err_bits[k % LPS_MAX_ITER_SLOTS] = 0u;
+        sg->final_bits = eb;
         if (eb >= 0x7F800000u) {
             atomicOr(&s_nan, 1u);
             continue;
         }
-        // non-negative floats order like their bit patterns
-        if (k == 1u || eb < sg->best_bits) {
-            sg->best_bits = eb;
-            sg->stall = 0u;
-        } else {
-            sg->stall += 1u;
+        const uint32_t cur = sg->buf_of[k % LPS_MAX_ITER_SLOTS];
+        // stall bookkeeping from the second correction on (the first one's
+        // predictions carry no pressure force); non-negative floats order
+        // like their bit patterns.  Correction k's predictions came from the
+        // buffer of correction k - 1.
+        if (k >= 2u) {
+            if (sg->best_k == 0u || eb < sg->best_bits) {
+                sg->best_bits = eb;
+                sg->best_k = k;
+                sg->best_buf = sg->buf_of[(k - 1u) % LPS_MAX_ITER_SLOTS];
+                sg->stall = 0u;
+            } else {
+                sg->stall += 1u;
+            }
         }
         const float ratio = __uint_as_float(eb) / rest_density;
-        bool stop = false;
+        bool stop = false, stalled = false;
         if (k >= rule.min_iter && ratio < rule.tol) {
             stop = true;
             sg->converged = 1u;
         } else if (k >= rule.min_iter && rule.stall_limit > 0u && sg->stall >= rule.stall_limit) {
-            stop = true;
+            stop = stalled = true;
         } else if (k >= rule.max_iter) {
             stop = true;
         }
-        if (stop) sg->stop_iter = k;
-        else atomicAdd(&s_run, 1u);
-        atomicMax(&s_err, eb);
+        uint32_t reported = eb;
+        if (stop) {
+            // a stall exit applies the pressure force whose predictions had
+            // the smallest error (stall >= 1, so that correction is an
+            // earlier one); any other stop applies this correction's force
+            sg->stop_iter = k;
+            sg->final_buf = stalled ? sg->best_buf : cur;
+            if (stalled) reported = sg->best_bits;
+            sg->final_bits = reported;
+        } else {
+            // correction k + 1 writes the buffer that is neither this
+            // correction's nor the best one's
+            const uint32_t keep = sg->best_k ? sg->best_buf : cur;
+            uint32_t nb = 0u;
+            while (nb == cur || nb == keep) nb++;
+            sg->buf_of[(k + 1u) % LPS_MAX_ITER_SLOTS] = (uint8_t)nb;
+            atomicAdd(&s_run, 1u);
+        }
+        atomicMax(&s_err, reported);
         if (!sg->converged) atomicAnd(&s_conv, 0u);
     }
     __syncthreads();
@@ -1429,9 +1467,10 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
         pressure(std::true_type{});
     else
         pressure(std::false_type{});
-    const float ri = rho_pred[i];
+    const size_t ob = (size_t)S.grid[B.s].buf_of[k % LPS_MAX_ITER_SLOTS] * mk.stride;
+    const float ri = rho_pred[ob + i];
     const float4 f4 = make_float4((-pfx) * c.mass * ri, (-pfy) * c.mass * ri, (-pfz) * c.mass * ri, 0.f);
-    fp[i] = f4;
+    fp[ob + i] = f4;
     const float4 f = force[i];
     const float4 xr = posrho[i];
     const float4 xn = predict(make_float4(xr.x, xr.y, xr.z, 0.f), vel[i], vh[i], first[i] != 0, f.x + f4.x,
@@ -1449,7 +1488,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     const float4* __restrict__ force, const float4* __restrict__ fp, const float4* __restrict__ posrho,
     float* __restrict__ pprev, const float* __restrict__ press, DevGrid* g, SimConst c0,
     const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
-    const lps_plane* __restrict__ planes0, uint32_t* __restrict__ bbox_part, lps_dev::Scenes S, uint32_t nlogical) {
+    const lps_plane* __restrict__ planes0, uint32_t* __restrict__ bbox_part, lps_dev::Scenes S, uint32_t stride,
+    uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const bool dead = step_dead(g);
@@ -1465,9 +1505,12 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
         const float4 x4 = pos[i];
         p = make_float3(x4.x, x4.y, x4.z);
         if (!dead) {
+            // PCISPH: the pressure force and pressure of the buffer the
+            // scene's loop chose; SPH: no pressure force here, buffer 0
+            const size_t ob = fp ? (size_t)S.grid[B.s].final_buf * stride : 0u;
             float4 f = force[i];
             if (fp) {
-                const float4 a = fp[i];
+                const float4 a = fp[ob + i];
                 f.x += a.x; f.y += a.y; f.z += a.z;
             }
             const float rho = posrho[i].w;
@@ -1488,7 +1531,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
             vh[i] = make_float4(hv.x, hv.y, hv.z, 0.f);
             pos[i] = make_float4(p.x, p.y, p.z, 0.f);
             vel[i] = make_float4((old.x + hv.x) / 2.f, (old.y + hv.y) / 2.f, (old.z + hv.z) / 2.f, 0.f);
-            if (pprev) pprev[i] = press[i];
+            if (pprev) pprev[i] = press[ob + i];
         }
     }
     bbox_partial(bbox_part, lb, valid, p.x, p.y, p.z);
@@ -1606,7 +1649,8 @@ void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float*
                       uint32_t* bbox_part) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_integrate<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.vel, a.vh, a.first, a.force, fp, a.posrho, pprev, press,
-                                                   a.g, a.c, a.boxes, a.spheres, a.planes, bbox_part, a.sc, nl);
+                                                   a.g, a.c, a.boxes, a.spheres, a.planes, bbox_part, a.sc,
+                                                   a.mk.stride, nl);
 }
 
 void launch_export_positions(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, float* dst,

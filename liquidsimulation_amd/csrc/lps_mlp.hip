@@ -271,6 +271,7 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
                                                         const float* __restrict__ feat, uint32_t fs, uint32_t n,
                                                         float* __restrict__ press, float* __restrict__ q,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
+                                                        const float* __restrict__ pprev, float kappa,
                                                         float pscale, int clamp_nonneg, const DevGrid* g,
                                                         lps_dev::Scenes sc, LoopRule rule, float rest_density) {
     if (g->status != 0u) return;
@@ -302,6 +303,8 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
             1u >= rule.max_iter)
             return;
         const uint32_t end = min(n, sc.info[s].slot0 + sc.info[s].count);
+        // the first correction's pressure buffer of this scene
+        const size_t ob = (size_t)sc.grid[s].buf_of[1] * fs;
         __syncthreads();
         for (uint32_t grp = 0; grp < 4; grp++) {
             const uint32_t p = bb + w * 64u + grp * 16u + (l & 15u);
@@ -309,11 +312,21 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
             if (p < end && (l >> 4) == 0u) {
                 float v = y * tscale * pscale;
                 if (!isfinite(v)) v = 0.f;
-                if (clamp_nonneg) v = fmaxf(v, 0.f);
-                const float rp = rho_pred[p];
-                press[p] = v;
-                q[p] = v / (rp * rp);
                 seed[p] = v;
+                if (clamp_nonneg) {
+                    // With negative pressures disabled the loop never lowers
+                    // a pressure, so the seed stays within an envelope: at
+                    // least the zero start's own first-correction pressure
+                    // p1 (which the loop would reach anyway), at most kappa
+                    // times the particle's previous applied pressure unless
+                    // p1 is larger.  An over-prediction then shrinks by kappa
+                    // per step instead of feeding itself through the
+                    // previous-pressure feature.
+                    v = fmaxf(press[ob + p], fminf(v, kappa * pprev[p]));
+                }
+                const float rp = rho_pred[ob + p];
+                press[ob + p] = v;
+                q[p] = v / (rp * rp);
             }
         }
     }
@@ -519,12 +532,14 @@ void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed) {
 }
 
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
-                     float* press, float* q, float* seed, const float* rho_pred, float pscale, int clamp_nonneg,
-                     const DevGrid* g, const Scenes& sc, const LoopRule& rule, float rest_density) {
+                     float* press, float* q, float* seed, const float* rho_pred, const float* pprev, float kappa,
+                     float pscale, int clamp_nonneg, const DevGrid* g, const Scenes& sc, const LoopRule& rule,
+                     float rest_density) {
     // one block per particle block (n is a whole number of blocks)
     k_mlp_seed<<<(n + LPS_BLOCK - 1u) / LPS_BLOCK, LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale,
-                                                                      feat, fs, n, press, q, seed, rho_pred, pscale,
-                                                                      clamp_nonneg, g, sc, rule, rest_density);
+                                                                      feat, fs, n, press, q, seed, rho_pred, pprev,
+                                                                      kappa, pscale, clamp_nonneg, g, sc, rule,
+                                                                      rest_density);
 }
 
 void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y) {

@@ -343,7 +343,8 @@ class PCISPHSolver(SPHSolver):
         check(lib().lps_mlp_capture(self._ctx, 1 if on else 0), "lps_mlp_capture")
 
     def dump_pairs(self, seed: bool = False):
-        """Features [n, 16], final pressure [n] (and the seed [n]) of the last step."""
+        """Features [n, 16], applied pressure [n] (and the network's seed [n],
+        before the envelope) of the last step."""
         n = self._n
         x = np.zeros((n, LPS_MLP_IN), np.float32)
         t = np.zeros(n, np.float32)

@@ -528,9 +528,27 @@ inline V3<T> cell_anchor(const std::vector<V3<float>>& xf, int32_t i) {
     return V3<T>{(T)xf[i].x, (T)xf[i].y, (T)xf[i].z};
 }
 
+// Test hook: 0 makes the stall exit apply the last correction (the rule's
+// earlier form, kept so a test can show what the best-correction exit fixes).
+static int g_best_exit = 1;
+inline bool best_exit_enabled() { return g_best_exit != 0; }
+
+// Learned warm start of a step (no reference counterpart).  `seed` is the
+// network's pressure per particle in this step's sorted order (NULL: no
+// network, an unbounded value).  With `pprev` (the previous step's applied
+// pressure, incoming particle order) and negative pressures disabled, the
+// first correction's pressure becomes max(p1, min(seed, kappa * pprev)),
+// p1 the zero start's own first-correction pressure; otherwise it becomes
+// the seed.
+struct WarmStart {
+    const float* seed = nullptr;
+    const float* pprev = nullptr;
+    float kappa = 0.f;
+};
+
 template <class T>
 int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* x, T* v, T* vh,
-             uint8_t* first, orc_trace* tr, int nthreads, const float* seed = nullptr) {
+             uint8_t* first, orc_trace* tr, int nthreads, const WarmStart* warm = nullptr) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -688,8 +706,19 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
         int32_t minit = p->min_iterations;
         int32_t k = 0;
         bool nonfinite = false;
+        // stall rule and its exit (no reference counterpart): from correction
+        // 2 on, the correction kb with the smallest maximum error is
+        // remembered together with what its predictions came from (the
+        // pressure force, pressure and predicted density of correction
+        // kb - 1).  A loop stopped by the stall rule applies those; every
+        // other stop applies the last correction's, as the reference does.
         T best = 0;
-        int32_t stall = 0, converged = 0;
+        int32_t kb = 0;
+        std::vector<V3<T>> fp_best;
+        std::vector<V3<T>> fp_prev;
+        std::vector<T> p_best, rp_best, p_prev, rp_prev;
+        const bool keep_best = p->stall_iterations > 0 && best_exit_enabled();
+        int32_t stall = 0, converged = 0, stalled = 0;
         for (k = 1; k <= maxit; k++) {
             // pciIntegrate + pciHandleCollisions (SPHKernels.cl:676-735)
 #pragma omp parallel for schedule(static)
@@ -708,6 +737,11 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
                 pos.x += hv.x * dt; pos.y += hv.y * dt; pos.z += hv.z * dt;
                 collide_all_ocl(pos, hv, p, cl);
                 s.xp[i] = pos;
+            }
+            if (keep_best && k >= 2) {
+                p_prev = pp;
+                rp_prev = s.rp;
+                fp_prev = s.fp;
             }
             // pciCalcDensityPressure (SPHKernels.cl:737-804)
 #pragma omp parallel for schedule(dynamic, 256)
@@ -732,7 +766,7 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
                 pp[i] += dp;
                 err[i] = std::fabs(e);
             }
-            if (seed && k == 1) {
+            if (warm && k == 1) {
                 // learned warm start (no reference counterpart): unless the
                 // loop stops at the first correction, its pressure is replaced
                 // by the given seed (sorted order) before the pressure force
@@ -745,8 +779,18 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
                 float r1 = (float)mx1 / p->rest_density;
                 if (sizeof(T) == sizeof(double)) r1 = (float)(mx1 / rho0);
                 const bool stop1 = nf1 || (k >= minit && r1 < p->max_density_error_ratio) || k >= maxit;
-                if (!stop1)
-                    for (int32_t i = 0; i < n; i++) pp[i] = (T)seed[i];
+                if (!stop1) {
+                    const bool envelope = warm->pprev && npf == (T)0;
+                    for (int32_t i = 0; i < n; i++) {
+                        if (!envelope) {
+                            pp[i] = (T)warm->seed[i];
+                            continue;
+                        }
+                        const T nn = warm->seed ? (T)warm->seed[i] : (T)INFINITY;
+                        const T cap = (T)warm->kappa * (T)warm->pprev[g.perm[i]];
+                        pp[i] = std::max(pp[i], std::min(nn, cap));
+                    }
+                }
             }
             // pciCalcPressureForce (SPHKernels.cl:806-878)
 #pragma omp parallel for schedule(dynamic, 256)
@@ -781,22 +825,41 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
             float ratio = (float)mx / p->rest_density;
             if (sizeof(T) == sizeof(double)) ratio = (float)(mx / rho0);
             if (tr && k <= 64) tr->err_ratio[k - 1] = ratio;
+            if (tr) tr->final_err_ratio = ratio;
             if (nonfinite) break;
-            // stall rule (no reference counterpart): count corrections that do
-            // not improve on the best maximum error so far
-            if (k == 1 || mx < best) {
-                best = mx;
-                stall = 0;
-            } else {
-                stall++;
+            // stall rule: count corrections that do not improve on the best
+            // maximum error so far (from correction 2 on)
+            if (k >= 2) {
+                if (kb == 0 || mx < best) {
+                    best = mx;
+                    kb = k;
+                    stall = 0;
+                    if (keep_best) {
+                        fp_best = fp_prev;
+                        p_best = p_prev;
+                        rp_best = rp_prev;
+                    }
+                } else {
+                    stall++;
+                }
             }
             if (k >= minit && ratio < p->max_density_error_ratio) {
                 converged = 1;
                 break;
             }
-            if (k >= minit && p->stall_iterations > 0 && stall >= p->stall_iterations) break;
+            if (k >= minit && p->stall_iterations > 0 && stall >= p->stall_iterations) {
+                stalled = 1;
+                break;
+            }
         }
         if (k > maxit) k = maxit;
+        if (stalled && keep_best) {
+            s.fp = fp_best;
+            pp = p_best;
+            s.rp = rp_best;
+            if (tr) tr->final_err_ratio = (float)(best / rho0);
+            if (tr && sizeof(T) == sizeof(float)) tr->final_err_ratio = (float)best / p->rest_density;
+        }
         if (tr) {
             tr->iterations = k;
             tr->converged = converged;
@@ -1090,6 +1153,7 @@ int step_seq_f32(const orc_params* p, const orc_box* boxes, const orc_sphere* sp
             for (int32_t i = 0; i < n; i++) mx = std::max(mx, std::fabs(de[i]));
             float ratio = mx / rho0;
             if (tr && k <= 64) tr->err_ratio[k - 1] = ratio;
+            if (tr) tr->final_err_ratio = ratio;
             if (k >= p->min_iterations && std::fabs(ratio) < p->max_density_error_ratio) break;
         }
         if (k > maxit) k = maxit;
@@ -1136,6 +1200,8 @@ int step_seq_f32(const orc_params* p, const orc_box* boxes, const orc_sphere* sp
 
 // ================================================================= C API ==
 extern "C" {
+
+void orc_set_best_exit(int32_t on) { g_best_exit = on; }
 
 int orc_derived_f32(const orc_params* p, float* h, float* div_step, float* mass, float* tables) {
     Tables<float> t = make_tables<float>(kernel_radius<float>(p));
@@ -1192,7 +1258,22 @@ int orc_step_ocl_seeded_f32(const orc_params* p, const orc_box* boxes, int32_t n
                             int32_t nthreads) {
     if (!seed) return ST_INVALID;
     Colliders cl{boxes, nbox, spheres, nsphere, planes, nplane};
-    return step_ocl<float>(p, cl, dt, n, x, v, vh, first, tr, nthreads, seed);
+    WarmStart w;
+    w.seed = seed;
+    return step_ocl<float>(p, cl, dt, n, x, v, vh, first, tr, nthreads, &w);
+}
+
+int orc_step_ocl_warm_f32(const orc_params* p, const orc_box* boxes, int32_t nbox, const orc_sphere* spheres,
+                          int32_t nsphere, const orc_plane* planes, int32_t nplane, float dt, int32_t n, float* x,
+                          float* v, float* vh, uint8_t* first, const float* seed, const float* pprev, float kappa,
+                          orc_trace* tr, int32_t nthreads) {
+    if (!pprev) return ST_INVALID;
+    Colliders cl{boxes, nbox, spheres, nsphere, planes, nplane};
+    WarmStart w;
+    w.seed = seed;
+    w.pprev = pprev;
+    w.kappa = kappa;
+    return step_ocl<float>(p, cl, dt, n, x, v, vh, first, tr, nthreads, &w);
 }
 
 int orc_step_ocl_f64(const orc_params* p, const orc_box* boxes, int32_t nbox, const orc_sphere* spheres,
@@ -1256,7 +1337,7 @@ int orc_run_ocl_f32(const orc_params* p, const orc_box* boxes, int32_t nbox, con
         std::memset(&tr, 0, sizeof(tr));
         int st = step_ocl<float>(p, cl, dt, n, x, v, vh, first, &tr, nthreads);
         if (iterations_out) iterations_out[s] = tr.iterations;
-        if (err_out) err_out[s] = tr.iterations > 0 && tr.iterations <= 64 ? tr.err_ratio[tr.iterations - 1] : 0.f;
+        if (err_out) err_out[s] = tr.iterations > 0 ? tr.final_err_ratio : 0.f;
         if (st != ST_OK) return st;
     }
     return ST_OK;

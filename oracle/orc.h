@@ -93,6 +93,7 @@ typedef struct orc_trace {
     float err_ratio[64];   /* max density error ratio per iteration (first 64) */
     int32_t converged;     /* PCISPH: tolerance met */
     int32_t status;
+    float final_err_ratio; /* max density error ratio of the last correction */
 } orc_trace;
 
 /* derived quantities (SPHSolver::recalcKernelRadius / recalcParticleMass) */
@@ -118,6 +119,16 @@ int orc_step_ocl_seeded_f32(const orc_params* p, const orc_box* boxes, int32_t n
                             const orc_plane* planes, int32_t nplane,
                             float dt, int32_t n, float* x, float* v, float* vh, uint8_t* first,
                             const float* seed, orc_trace* tr, int32_t nthreads);
+/* the learned warm start's seed rule: with negative pressures disabled, the
+ * first correction's pressure becomes max(p1, min(seed, kappa * pprev)),
+ * p1 the zero start's first-correction pressure, seed[n] the network's
+ * pressure in this step's sorted order (NULL: unbounded) and pprev[n] the
+ * previous step's applied pressure in the incoming particle order */
+int orc_step_ocl_warm_f32(const orc_params* p, const orc_box* boxes, int32_t nbox,
+                          const orc_sphere* spheres, int32_t nsphere,
+                          const orc_plane* planes, int32_t nplane,
+                          float dt, int32_t n, float* x, float* v, float* vh, uint8_t* first,
+                          const float* seed, const float* pprev, float kappa, orc_trace* tr, int32_t nthreads);
 int orc_step_ocl_f64(const orc_params* p, const orc_box* boxes, int32_t nbox,
                      const orc_sphere* spheres, int32_t nsphere,
                      const orc_plane* planes, int32_t nplane,
@@ -137,6 +148,10 @@ int orc_collide_ocl(const orc_params* p, const orc_box* boxes, int32_t nbox,
                     int32_t n, float* x, float* vel);
 int orc_neighbors_seq(float h, int32_t n, const float* x, int32_t i, int32_t* out, int32_t cap);
 int orc_brute_density(const orc_params* p, int32_t n, const float* x, double* rho);
+
+/* test hook: 0 makes the stall exit apply the last correction instead of the
+ * best one (the stall rule's earlier form); default 1 */
+void orc_set_best_exit(int32_t on);
 
 /* multi-step drivers for the CPU baseline (ocl flavour, OpenMP) */
 int orc_run_ocl_f32(const orc_params* p, const orc_box* boxes, int32_t nbox,

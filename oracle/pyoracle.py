@@ -48,6 +48,7 @@ class COrcTrace(C.Structure):
         ("err_ratio", C.c_float * 64),
         ("converged", C.c_int32),
         ("status", C.c_int32),
+        ("final_err_ratio", C.c_float),
     ]
 
 
@@ -82,6 +83,9 @@ def lib():
         L.orc_step_ocl_f64.argtypes = step_args + [_f64p, _f64p, _f64p, _u8p, C.POINTER(COrcTrace), C.c_int32]
         L.orc_step_ocl_seeded_f32.argtypes = step_args + [_f32p, _f32p, _f32p, _u8p, C.c_void_p,
                                                           C.POINTER(COrcTrace), C.c_int32]
+        L.orc_step_ocl_warm_f32.argtypes = step_args + [_f32p, _f32p, _f32p, _u8p, C.c_void_p, _f32p, C.c_float,
+                                                        C.POINTER(COrcTrace), C.c_int32]
+        L.orc_set_best_exit.argtypes = [C.c_int32]
         L.orc_step_seq_f32.argtypes = [C.POINTER(CParams), C.c_void_p, C.c_void_p, C.c_void_p, _i32p, _i32p,
                                        C.c_int32, C.c_float, C.c_int32, _f32p, _f32p, _f32p, _u8p,
                                        C.POINTER(COrcTrace)]
@@ -186,6 +190,7 @@ class Trace:
     delta: float = 0.0
     err_ratio: Optional[np.ndarray] = None
     converged: int = 0
+    final_err_ratio: float = 0.0
 
 
 def _mk_trace(n, cell_cap, fdtype):
@@ -211,6 +216,7 @@ def _read_trace(tr, bufs, n) -> Trace:
               grid_offset=np.array(tr.grid_offset[:], np.float32), grid_spacing=tr.grid_spacing,
               grid_size=np.array(tr.grid_size[:], np.uint32), cell_count=tr.cell_count,
               iterations=tr.iterations, delta=tr.delta, converged=tr.converged,
+              final_err_ratio=tr.final_err_ratio,
               err_ratio=np.array(tr.err_ratio[:max(0, min(tr.iterations, 64))], np.float32))
     t.keys = bufs["keys"]
     t.perm = bufs["perm"]
@@ -264,6 +270,25 @@ def step_ocl_seeded(params: Params, col: Colliders, dt: float, st: State, seed: 
     x, v, vh = (a.reshape(-1) for a in (st.x, st.v, st.vh))
     lib().orc_step_ocl_seeded_f32(C.byref(cp), *cargs, dt, n, x, v, vh, st.first, seed.ctypes.data,
                                   C.byref(tr), nthreads)
+    del keep
+    return _read_trace(tr, bufs, n)
+
+
+def step_ocl_warm(params: Params, col: Colliders, dt: float, st: State, pprev: np.ndarray, kappa: float,
+                  seed: Optional[np.ndarray] = None, nthreads: int = 0, cell_cap: int = 1 << 22) -> Trace:
+    """step_ocl with the warm start's seed rule: the first correction's
+    pressure becomes max(p1, min(seed, kappa * pprev)) (negative pressures
+    disabled), `pprev` in the incoming particle order, `seed` in this step's
+    sorted order (None: no network bound)."""
+    n = st.n
+    pprev = np.ascontiguousarray(pprev, np.float32).reshape(n)
+    sp = None if seed is None else np.ascontiguousarray(seed, np.float32).reshape(n)
+    tr, bufs = _mk_trace(n, cell_cap, np.float32)
+    cp = params.to_c()
+    keep, cargs = _col_ptrs(col)
+    x, v, vh = (a.reshape(-1) for a in (st.x, st.v, st.vh))
+    lib().orc_step_ocl_warm_f32(C.byref(cp), *cargs, dt, n, x, v, vh, st.first,
+                                None if sp is None else sp.ctypes.data, pprev, kappa, C.byref(tr), nthreads)
     del keep
     return _read_trace(tr, bufs, n)
 

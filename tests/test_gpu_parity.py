@@ -177,3 +177,40 @@ def test_obstacles_2m_3_steps_bit_exact(gpu, oracle):
     for k, ref in (("x", a.x), ("v", a.v), ("vhalf", a.vh)):
         assert ulp_diff(p[k], ref) == 0, k
     np.testing.assert_array_equal(p["first"], a.first)
+
+
+def test_corrections_past_64_use_their_own_error(gpu, oracle):
+    """The device keeps 64 per-correction error slots as a ring.  A loop that
+    runs past 64 corrections must judge correction k by its own maximum error,
+    as the oracle does, not by a maximum carried over from earlier corrections.
+    The tolerance is set between the smallest error of corrections 3..64 and
+    the smallest of 65..80, so the oracle's loop converges past 64; GPU and
+    oracle must agree on the count, the final error and the state."""
+    sc = scenes.small_block(10, 12, 9, dt=0.002, max_iterations=80, stall_iterations=0,
+                            max_density_error_ratio=0.0)
+    st = oracle.State.from_scene(sc.x)
+    st.v[:, 1] = np.float32(-0.5)
+    errs = []
+    for kmax in range(1, 81):
+        tr = oracle.step_ocl(sc.params.replace(max_iterations=kmax), sc.colliders, sc.dt, st.copy())
+        assert tr.iterations == kmax
+        errs.append(tr.final_err_ratio)
+    errs = np.array(errs, np.float64)
+    lo, hi = float(errs[64:].min()), float(errs[2:64].min())
+    assert lo < hi
+    p = sc.params.replace(max_density_error_ratio=float(np.float32(0.5 * (lo + hi))))
+    for params in (sc.params, p):
+        a = st.copy()
+        tr = oracle.step_ocl(params, sc.colliders, sc.dt, a)
+        s, stats = gpu_run(params, sc.colliders, sc.dt, st, 1)
+        q = s.get_particles()
+        print(f"oracle its={tr.iterations} conv={tr.converged} err={tr.final_err_ratio:.6g}; gpu its="
+              f"{stats[0].iterations} conv={stats[0].converged} err={stats[0].max_density_error_ratio:.6g}",
+              flush=True)
+        assert tr.iterations > 64
+        assert stats[0].iterations == tr.iterations
+        assert stats[0].converged == tr.converged
+        assert np.float32(stats[0].max_density_error_ratio) == np.float32(tr.final_err_ratio)
+        assert ulp_diff(q["x"], a.x) == 0
+        assert ulp_diff(q["v"], a.v) == 0
+        s.close()
