@@ -285,6 +285,19 @@ int lps_mlp_forward(lps_ctx* ctx, const float* x /* [n*16] */, uint32_t n, float
 int lps_mlp_grad(lps_ctx* ctx, const float* x /* [n*16] */, const float* t /* [n] */, uint32_t n,
                  float over_weight, float* grad, float* loss);
 
+/* lps_mlp_grad without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight training or serving loops, harnesses that forbid hipMalloc).
+ * Same inputs, outputs and result as lps_mlp_grad, for n <= LPS_NOALLOC_MAX_ROWS
+ * rows (larger n: LPS_ERR_INVALID).  The staged inputs and the gradient
+ * accumulators live in one static device array per GPU (about 4.3 MB, held
+ * until the process ends) shared by every call on that GPU: calls that
+ * overlap on one GPU (two threads, two contexts) race.  Needs weights loaded
+ * (LPS_ERR_STATE otherwise); the context's stream is synchronised on return. */
+#define LPS_NOALLOC_MAX_ROWS 65536
+int lps_mlp_grad_noalloc(lps_ctx* ctx, const float* x /* [n*16] */, const float* t /* [n] */, uint32_t n,
+                         float over_weight, float* grad, float* loss);
+
 /* Kernel timing (HIP events on the context stream), for measurement only. */
 int lps_profile_enable(lps_ctx* ctx, int32_t on);
 int lps_profile_query(lps_ctx* ctx, const char* kernel, uint32_t* launches, double* total_ms);

@@ -21,6 +21,8 @@ METHOD_PCISPH = 1
 WARM_NONE = 0
 WARM_LEARNED = 1
 WARM_LEARNED_ALWAYS = 2
+# rows accepted by lps_mlp_grad_noalloc
+LPS_NOALLOC_MAX_ROWS = 65536
 # seed envelope factor of the learned warm start (LPS_WARM_KAPPA, a float)
 WARM_KAPPA = 0.9
 

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -25,7 +26,42 @@
 #include "lps_common.h"
 #include "lps_kernels.h"
 
+// Scratch memory of lps_mlp_grad_noalloc: one array per GPU, held until the
+// process ends.  Layout (bytes): staged features [LPS_NOALLOC_MAX_ROWS][16]
+// floats, staged targets [LPS_NOALLOC_MAX_ROWS] floats, gradient
+// accumulators [LPS_MLP_PARAMS] floats (padded to 256 bytes), loss [1].
+#define LPS_NOALLOC_X_BYTES ((size_t)LPS_NOALLOC_MAX_ROWS * LPS_MLP_IN * sizeof(float))
+#define LPS_NOALLOC_T_BYTES ((size_t)LPS_NOALLOC_MAX_ROWS * sizeof(float))
+#define LPS_NOALLOC_G_BYTES ((((size_t)LPS_MLP_PARAMS * sizeof(float)) + 255u) / 256u * 256u)
+#define LPS_NOALLOC_BYTES (LPS_NOALLOC_X_BYTES + LPS_NOALLOC_T_BYTES + LPS_NOALLOC_G_BYTES + 256u)
+static __device__ __align__(256) unsigned char scratch[LPS_NOALLOC_BYTES];
+
 namespace {
+
+// Device address of `scratch` per device ordinal, resolved once per device
+// with that device current (the host-side name is only a stand-in).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+int g_scratch_status[kMaxDevices] = {};  // 0 unresolved, 1 resolved, 2 failed
+
+// Called by lps_create with the context's device current.
+void resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_status[device] != 0) return;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+    g_scratch_status[device] = (e == hipSuccess && ptr) ? 1 : 2;
+    if (e != hipSuccess) (void)hipGetLastError();
+}
+
+unsigned char* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_status[device] == 1 ? static_cast<unsigned char*>(g_scratch_ptr[device]) : nullptr;
+}
 
 struct ProfRec {
     const char* name;
@@ -786,6 +822,7 @@ int lps_create(const lps_params* params, lps_ctx** out) {
     c->params = *params;
     lps_compute_derived(&c->params, &c->der);
     (void)hipGetDevice(&c->device);
+    resolve_scratch(c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return LPS_ERR_HIP;
@@ -1741,6 +1778,37 @@ int lps_mlp_grad(lps_ctx* c, const float* x, const float* t, uint32_t n, float o
     dfree(dx);
     dfree(dt);
     return st;
+}
+
+int lps_mlp_grad_noalloc(lps_ctx* c, const float* x, const float* t, uint32_t n, float over_weight, float* grad,
+                         float* loss) {
+    if (!c || !x || !t || !grad || n == 0 || !(over_weight > 0.f) || !std::isfinite(over_weight))
+        return LPS_ERR_INVALID;
+    if (n > LPS_NOALLOC_MAX_ROWS) return LPS_ERR_INVALID;
+    if (!c->mlp_loaded) return LPS_ERR_STATE;
+    unsigned char* base = scratch_ptr(c->device);
+    if (!base) return LPS_ERR_HIP;
+    float* dx = reinterpret_cast<float*>(base);
+    float* dt = reinterpret_cast<float*>(base + LPS_NOALLOC_X_BYTES);
+    float* dg = reinterpret_cast<float*>(base + LPS_NOALLOC_X_BYTES + LPS_NOALLOC_T_BYTES);
+    float* dl = reinterpret_cast<float*>(base + LPS_NOALLOC_X_BYTES + LPS_NOALLOC_T_BYTES + LPS_NOALLOC_G_BYTES);
+    const hipStream_t s = c->stream;
+    // the accumulators keep the previous call's sums: clear them (a fill
+    // kernel, so the call stays free of memset nodes)
+    lps_dev::launch_fill_zero(s, dg, LPS_MLP_PARAMS);
+    lps_dev::launch_fill_zero(s, dl, 1u);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(dx, x, sizeof(float) * LPS_MLP_IN * (size_t)n, hipMemcpyHostToDevice, s));
+    HIPC(hipMemcpyAsync(dt, t, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    const lps_dev::RowMap rm{nullptr, 0u, 1u, 0u, n};
+    lps_dev::launch_mlp_grad(s, mlp_args(c), dx, dt, rm, n, over_weight, dg, dl);
+    HIPC(hipGetLastError());
+    float l = 0.f;
+    HIPC(hipMemcpyAsync(grad, dg, sizeof(float) * LPS_MLP_PARAMS, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(&l, dl, sizeof(float), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (loss) *loss = l;
+    return LPS_OK;
 }
 
 int lps_profile_enable(lps_ctx* c, int32_t on) {

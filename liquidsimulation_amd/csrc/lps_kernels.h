@@ -134,6 +134,8 @@ size_t mlp_packed_bytes();
 void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed);
 // batch inference (raw output, before tscale) / gradients, for training and tests
 void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y);
+// p[0..n) = 0 with a kernel (no memset node)
+void launch_fill_zero(hipStream_t s, float* p, uint32_t n);
 void launch_mlp_grad(hipStream_t s, const MlpArgs& m, const float* x, const float* t, const RowMap& rm, uint32_t n,
                      float over_weight, float* grad, float* loss);
 void launch_adam(hipStream_t s, float* params, float* grad, float* m1, float* m2, uint32_t count, float lr,

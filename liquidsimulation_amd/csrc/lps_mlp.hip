@@ -333,6 +333,12 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
 }
 
 
+// p[0..n) = 0 (clears accumulators without a memset node)
+__global__ void k_fill_zero(float* __restrict__ p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
 // Gradient of the weighted mean squared error mean(w d^2), d = y - t/tscale,
 // w = over_weight where d > 0 and 1 elsewhere, over a batch of n rows of x / t
 // (row of sample s given by rm) with respect to the FP32 parameters, added to
@@ -540,6 +546,11 @@ void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const floa
                                                                       feat, fs, n, press, q, seed, rho_pred, pprev,
                                                                       kappa, pscale, clamp_nonneg, g, sc, rule,
                                                                       rest_density);
+}
+
+void launch_fill_zero(hipStream_t s, float* p, uint32_t n) {
+    if (n == 0) return;
+    k_fill_zero<<<(n + 255u) / 256u, 256, 0, s>>>(p, n);
 }
 
 void launch_mlp_forward(hipStream_t s, const void* packed, const float* x, uint32_t n, float* y) {

@@ -385,6 +385,17 @@ class PCISPHSolver(SPHSolver):
               "lps_mlp_grad")
         return g, loss.value
 
+    def mlp_grad_noalloc(self, x, t, over_weight: float = 1.0):
+        """mlp_grad through lps_mlp_grad_noalloc (no device allocation; at most
+        LPS_NOALLOC_MAX_ROWS rows)."""
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, LPS_MLP_IN)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1)
+        g = np.zeros(LPS_MLP_PARAMS, np.float32)
+        loss = C.c_float()
+        check(lib().lps_mlp_grad_noalloc(self._ctx, x.reshape(-1), t, x.shape[0], float(over_weight), g,
+                                         C.byref(loss)), "lps_mlp_grad_noalloc")
+        return g, loss.value
+
     def setMinIterations(self, k: int):
         self._apply(min_iterations=int(k))
 

@@ -29,6 +29,7 @@ int main(void) {
   printf("lps_grid_info %zu\n", sizeof(lps_grid_info));
   printf("mlp_params %d\n", LPS_MLP_PARAMS);
   printf("abi %u\n", LPS_ABI_VERSION);
+  printf("noalloc_rows %d\n", LPS_NOALLOC_MAX_ROWS);
   { union { float f; unsigned u; } k; k.f = LPS_WARM_KAPPA; printf("kappa_bits %u\n", k.u); }
   F(lps_params, gravity) F(lps_params, particle_radius) F(lps_params, mass_override)
   F(lps_params, min_iterations) F(lps_params, max_density_error_ratio) F(lps_params, warm_start)
@@ -51,6 +52,11 @@ def c_layout():
         subprocess.check_call(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", exe, src])
         out = subprocess.check_output([exe], text=True)
     return {ln.split()[0]: int(ln.split()[1]) for ln in out.splitlines()}
+
+
+def test_noalloc_rows_match_header(c_layout):
+    from liquidsimulation_amd.abi import LPS_NOALLOC_MAX_ROWS
+    assert c_layout["noalloc_rows"] == LPS_NOALLOC_MAX_ROWS
 
 
 def test_warm_kappa_matches_header(c_layout):
