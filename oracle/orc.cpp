@@ -870,6 +870,18 @@ int step_ocl(const orc_params* p, const Colliders& cl, float dtf, int32_t n, T* 
         }
         if (tr && tr->pressure) for (int32_t i = 0; i < n; i++) ((T*)tr->pressure)[i] = pp[i];
         if (tr && tr->rho_pred) for (int32_t i = 0; i < n; i++) ((T*)tr->rho_pred)[i] = s.rp[i];
+        if (tr && tr->xpred)
+            for (int32_t i = 0; i < n; i++) {
+                ((T*)tr->xpred)[3 * i] = s.xp[i].x;
+                ((T*)tr->xpred)[3 * i + 1] = s.xp[i].y;
+                ((T*)tr->xpred)[3 * i + 2] = s.xp[i].z;
+            }
+        if (tr && tr->fp)
+            for (int32_t i = 0; i < n; i++) {
+                ((T*)tr->fp)[3 * i] = s.fp[i].x;
+                ((T*)tr->fp)[3 * i + 1] = s.fp[i].y;
+                ((T*)tr->fp)[3 * i + 2] = s.fp[i].z;
+            }
         if (nonfinite) {
             if (tr) tr->status = ST_NONFINITE;
         }
@@ -1163,6 +1175,18 @@ int step_seq_f32(const orc_params* p, const orc_box* boxes, const orc_sphere* sp
         }
         if (tr && tr->pressure) std::memcpy(tr->pressure, pr.data(), sizeof(T) * n);
         if (tr && tr->rho_pred) std::memcpy(tr->rho_pred, rp.data(), sizeof(T) * n);
+        if (tr && tr->xpred)
+            for (int32_t i = 0; i < n; i++) {
+                ((T*)tr->xpred)[3 * i] = xp[i].x;
+                ((T*)tr->xpred)[3 * i + 1] = xp[i].y;
+                ((T*)tr->xpred)[3 * i + 2] = xp[i].z;
+            }
+        if (tr && tr->fp)
+            for (int32_t i = 0; i < n; i++) {
+                ((T*)tr->fp)[3 * i] = fp[i].x;
+                ((T*)tr->fp)[3 * i + 1] = fp[i].y;
+                ((T*)tr->fp)[3 * i + 2] = fp[i].z;
+            }
     }
     if (tr && tr->force)
         for (int32_t i = 0; i < n; i++) {

@@ -29,6 +29,17 @@
  *   integrate / collisions  -> free-fall closed form, per-case hand computations,
  *                              oracle/_ref (reference SPHParticle / StaticCollision*)
  *   PCISPH loop             -> iteration-count rule, monotone error on a compressed block
+ *   PCISPH corrections      -> prediction, predicted density, pressure update and
+ *                              pressure force against FP64 brute force over all pairs
+ *                              with the 27-cell rule of the current cells (including
+ *                              a state where that anchor decides pairs), momentum of
+ *                              the pressure pair terms, OpenCL vs sequential form once
+ *                              the rho*_i / rho_i prefactor (H2) is divided out
+ *                              (tests/test_oracle_pcisph.py)
+ *   stall rule and its exit -> energy of the 16k dam break with the best-correction
+ *                              exit vs the last-correction exit (test hook), and the
+ *                              524k dam break's error floor at the walls
+ *                              (tests/test_oracle_stability.py)
  *   plane collider          -> no reference counterpart (new collider), pinned by
  *                              hand-computed cases only
  */
@@ -94,6 +105,8 @@ typedef struct orc_trace {
     int32_t converged;     /* PCISPH: tolerance met */
     int32_t status;
     float final_err_ratio; /* max density error ratio of the last correction */
+    void* xpred;           /* [3n] PCISPH: predictions of the last correction */
+    void* fp;              /* [3n] PCISPH: the pressure force the step applied */
 } orc_trace;
 
 /* derived quantities (SPHSolver::recalcKernelRadius / recalcParticleMass) */

@@ -49,6 +49,8 @@ class COrcTrace(C.Structure):
         ("converged", C.c_int32),
         ("status", C.c_int32),
         ("final_err_ratio", C.c_float),
+        ("xpred", C.c_void_p),
+        ("fp", C.c_void_p),
     ]
 
 
@@ -186,6 +188,8 @@ class Trace:
     pressure: Optional[np.ndarray] = None
     force: Optional[np.ndarray] = None
     rho_pred: Optional[np.ndarray] = None
+    xpred: Optional[np.ndarray] = None
+    fp: Optional[np.ndarray] = None
     iterations: int = 0
     delta: float = 0.0
     err_ratio: Optional[np.ndarray] = None
@@ -203,6 +207,8 @@ def _mk_trace(n, cell_cap, fdtype):
         "pressure": np.zeros(n, fdtype),
         "force": np.zeros(3 * n, fdtype),
         "rho_pred": np.zeros(n, fdtype),
+        "xpred": np.zeros(3 * n, fdtype),
+        "fp": np.zeros(3 * n, fdtype),
     }
     for k, a in bufs.items():
         setattr(tr, k, a.ctypes.data)
@@ -225,6 +231,8 @@ def _read_trace(tr, bufs, n) -> Trace:
     t.pressure = bufs["pressure"]
     t.force = bufs["force"].reshape(n, 3)
     t.rho_pred = bufs["rho_pred"]
+    t.xpred = bufs["xpred"].reshape(n, 3)
+    t.fp = bufs["fp"].reshape(n, 3)
     return t
 
 

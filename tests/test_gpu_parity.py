@@ -2,8 +2,19 @@
 
 Bar: keys, permutation and cell tables bit-exact; FP32 fields and the state
 after a step bit-exact as well (both sides evaluate the OpenCL kernels'
-expression orders in IEEE single precision without FMA contraction).  The
-error of the FP32 fields against the FP64 rerun is reported for information.
+expression orders in IEEE single precision without FMA contraction).
+
+Each single-step comparison also reports every field's error against the
+oracle's FP64 rerun of the same step (same tables and index rule, arithmetic
+in double): per element |fp32 - fp64| / max|fp64|, as max, 99th percentile
+and median.  The tails come from pairs whose FP32 and FP64 distances fall on
+the two sides of a step of the reference's piecewise-constant kernel tables
+(SPHKernel.cpp:19-20, SPHKernels.cl:237): one such pair moves a density by
+about 2e-4 rho0.  Where no pressure loop intervenes (the density, and every
+field of an SPH step) the median, the FP32 arithmetic error, must stay below
+1e-5; PCISPH's pressure updates (delta * e, with e of about 1e-2) turn each
+table step into a percent-level pressure change that the corrections spread,
+so its pressure-coupled fields are reported only.
 """
 import numpy as np
 import pytest
@@ -51,7 +62,31 @@ def _compare_step(oracle, sc, st, nsteps_before=0):
     np.testing.assert_array_equal(p["first"], a.first)
     for k, v in errs.items():
         assert v == 0, f"{k} differs by {v} ulp"
+    _fp64_report(oracle, sc, st, tr, f, p)
     return s, tr, a
+
+
+def _fp64_report(oracle, sc, st, tr, f, p):
+    """Error of the GPU's FP32 fields against the oracle's FP64 rerun."""
+    a64 = st.copy(np.float64)
+    t64 = oracle.step_ocl(sc.params, sc.colliders, sc.dt, a64)
+    assert t64.status == 0
+    np.testing.assert_array_equal(t64.perm, tr.perm)
+    fields = {"rho": (f["rho"], t64.rho), "pressure": (f["pressure"], t64.pressure),
+              "force": (f["force"], t64.force), "x": (p["x"], a64.x), "v": (p["v"], a64.v)}
+    if sc.params.method == METHOD_PCISPH:
+        fields["rho_pred"] = (f["rho_pred"], t64.rho_pred)
+    report = {}
+    for k, (g, ref) in fields.items():
+        ref = np.asarray(ref, np.float64)
+        scale = max(float(np.abs(ref).max()), 1e-300)
+        e = np.abs(np.asarray(g, np.float64) - ref).ravel() / scale
+        report[k] = (float(e.max()), float(np.percentile(e, 99)), float(np.median(e)))
+    print("  error vs FP64 rerun (max / p99 / median of |d| / max|fp64|):",
+          {k: "%.1e / %.1e / %.1e" % v for k, v in report.items()}, flush=True)
+    checked = report.keys() if sc.params.method == METHOD_SPH else ("rho",)
+    for k in checked:
+        assert report[k][2] <= 1e-5, f"{k}: median error {report[k][2]:.2e} against the FP64 rerun"
 
 
 @pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
