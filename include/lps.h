@@ -235,12 +235,18 @@ int lps_get_fields(lps_ctx* ctx, float* rho, float* pressure, float* force /* [3
  * With warm_start = LPS_WARM_LEARNED and weights
  * loaded the seed is used unless the running mean of corrections per step
  * with it exceeds the one without it (every 32nd step runs the other mode to
- * keep both current); a seeded step that turns non-finite is re-run unseeded. */
+ * keep both current); a seeded step that turns non-finite is re-run unseeded,
+ * and one that adds mechanical energy (LPS_WARM_ENERGY_TOL) switches the seed
+ * off until the next probe. */
 /* Seed envelope factor: with negative pressures disabled the first
  * correction's pressure is max(p1, min(seed, LPS_WARM_KAPPA * p_prev)), p1
  * the zero start's own first-correction pressure and p_prev the particle's
  * applied pressure of the previous step. */
-#define LPS_WARM_KAPPA 0.9f
+#define LPS_WARM_KAPPA 0.85f
+/* Energy tolerance of the fallback: a seeded step whose relative change of
+ * kinetic + potential energy (per scene, relative to |KE| + |PE|) exceeds
+ * the zero start's running mean (at least 0) by this much is a failed seed. */
+#define LPS_WARM_ENERGY_TOL 1e-4
 #define LPS_MLP_IN 16
 #define LPS_MLP_HIDDEN 64
 #define LPS_MLP_PARAMS (LPS_MLP_IN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN * LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + LPS_MLP_HIDDEN + 1)
@@ -297,6 +303,12 @@ int lps_mlp_grad(lps_ctx* ctx, const float* x /* [n*16] */, const float* t /* [n
 #define LPS_NOALLOC_MAX_ROWS 65536
 int lps_mlp_grad_noalloc(lps_ctx* ctx, const float* x /* [n*16] */, const float* t /* [n] */, uint32_t n,
                          float over_weight, float* grad, float* loss);
+
+/* Kinetic (sum m|v|^2/2) and potential (sum -m g.x) energy of every scene's
+ * current state, reduced on the device in double from per-block sums; no
+ * counterpart in the reference (long-run statistics, warm-start fallback). */
+int lps_get_energy(lps_ctx* ctx, double* kinetic /* [nscenes] */, double* potential /* [nscenes] */,
+                   uint32_t nscenes);
 
 /* Kernel timing (HIP events on the context stream), for measurement only. */
 int lps_profile_enable(lps_ctx* ctx, int32_t on);

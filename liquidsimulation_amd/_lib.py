@@ -81,6 +81,7 @@ def lib():
     L.lps_mlp_train.argtypes = [vp, C.POINTER(CTrainCfg), vp]
     L.lps_mlp_forward.argtypes = [vp, _f32p, C.c_uint32, _f32p]
     L.lps_mlp_grad.argtypes = [vp, _f32p, _f32p, C.c_uint32, C.c_float, _f32p, C.POINTER(C.c_float)]
+    L.lps_get_energy.argtypes = [vp, vp, vp, C.c_uint32]
     L.lps_mlp_grad_noalloc.argtypes = [vp, _f32p, _f32p, C.c_uint32, C.c_float, _f32p, C.POINTER(C.c_float)]
     L.lps_profile_enable.argtypes = [vp, C.c_int32]
     L.lps_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double)]

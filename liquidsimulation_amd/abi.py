@@ -24,7 +24,9 @@ WARM_LEARNED_ALWAYS = 2
 # rows accepted by lps_mlp_grad_noalloc
 LPS_NOALLOC_MAX_ROWS = 65536
 # seed envelope factor of the learned warm start (LPS_WARM_KAPPA, a float)
-WARM_KAPPA = 0.9
+WARM_KAPPA = 0.85
+# relative energy tolerance of the warm start's fallback (LPS_WARM_ENERGY_TOL)
+WARM_ENERGY_TOL = 1e-4
 
 STATUS_NAMES = {
     0: "LPS_OK",

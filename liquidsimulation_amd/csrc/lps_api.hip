@@ -122,6 +122,14 @@ struct lps_ctx {
     // host
     uint32_t* flags = nullptr;  // pinned, coherent: 4 words per iteration slot
     DevGrid* h_grid = nullptr;  // pinned copy
+    // coherent: per scene kinetic and potential energy of the state entering
+    // the last step (written by its grid set-up) or of the current state
+    // (lps_get_energy); the warm start's fallback compares consecutive steps
+    double* h_energy = nullptr;
+    std::vector<double> e_prev;  // per scene energy entering the previous step
+    bool e_prev_valid = false;
+    bool e_prev_seeded = false;
+    double ema_de_plain = 0.0;   // running relative energy change of zero-start steps
     int last_fp_valid = 0;      // PCISPH fields available
 
     // learned warm start: fp32 master weights, feature normalisation, target scale
@@ -393,8 +401,10 @@ void free_all(lps_ctx* c) {
     c->h_sgrid = nullptr;
     if (c->flags) (void)hipHostFree(c->flags);
     if (c->h_grid) (void)hipHostFree(c->h_grid);
+    if (c->h_energy) (void)hipHostFree(c->h_energy);
     c->flags = nullptr;
     c->h_grid = nullptr;
+    c->h_energy = nullptr;
 }
 
 lps_dev::Scenes scenes(const lps_ctx* c) {
@@ -433,7 +443,7 @@ void enqueue_grid(lps_ctx* c) {
     const uint32_t n = c->n;
     {
         Timed t(c, "grid_setup");
-        lps_dev::launch_grid_setup(s, c->grid, scenes(c), c->bbox_part, c->der.h, c->cell_cap);
+        lps_dev::launch_grid_setup(s, c->grid, scenes(c), c->bbox_part, c->der.h, c->cell_cap, c->h_energy);
     }
     {
         Timed t(c, "keys");
@@ -505,6 +515,40 @@ void ws_record(lps_ctx* c, bool seeded, double iterations) {
     e = e < 0.0 ? iterations : 0.7 * e + 0.3 * iterations;
     c->ws_on = c->ema_plain < 0.0 || c->ema_seeded < 0.0 || c->ema_seeded <= c->ema_plain;
     c->ws_steps++;
+}
+
+// Energy judgement of the previous step, once the grid set-up of the current
+// one has written the energy entering it (h_energy).  With negative pressures
+// disabled an over-predicted seed is never pulled back by the loop and shows
+// up as a step that adds mechanical energy; a seeded step whose relative
+// energy change exceeds that of the zero start (running mean, at least 0) by
+// LPS_WARM_ENERGY_TOL in any scene counts as a failed seed, like a non-finite
+// one.
+void ws_energy(lps_ctx* c, bool seeded_now) {
+    const size_t S = c->h_sinfo.size();
+    if (c->e_prev.size() != 2 * S) c->e_prev.assign(2 * S, 0.0);
+    if (c->e_prev_valid && warm_ready(c)) {
+        double worst = -INFINITY;
+        for (size_t k = 0; k < S; k++) {
+            const double e0 = c->e_prev[2 * k] + c->e_prev[2 * k + 1];
+            const double e1 = c->h_energy[2 * k] + c->h_energy[2 * k + 1];
+            const double scale = std::fabs(c->e_prev[2 * k]) + std::fabs(c->e_prev[2 * k + 1]);
+            if (scale > 0.0) worst = std::max(worst, (e1 - e0) / scale);
+        }
+        if (std::isfinite(worst)) {
+            if (c->e_prev_seeded) {
+                if (worst > std::max(c->ema_de_plain, 0.0) + (double)LPS_WARM_ENERGY_TOL) {
+                    c->ema_seeded = (double)c->params.max_iterations + 1.0;
+                    c->ws_on = false;
+                }
+            } else {
+                c->ema_de_plain = 0.7 * c->ema_de_plain + 0.3 * worst;
+            }
+        }
+    }
+    for (size_t k = 0; k < 2 * S; k++) c->e_prev[k] = c->h_energy[k];
+    c->e_prev_valid = true;
+    c->e_prev_seeded = seeded_now;
 }
 
 bool capture_features(const lps_ctx* c, bool seeded) {
@@ -831,6 +875,8 @@ int lps_create(const lps_params* params, lps_ctx** out) {
               hipHostMalloc((void**)&c->flags, sizeof(uint32_t) * 4 * LPS_MAX_ITER_SLOTS,
                             hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
               hipHostMalloc((void**)&c->h_grid, sizeof(DevGrid), hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc((void**)&c->h_energy, sizeof(double) * 2 * LPS_MAX_SCENES,
+                            hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
               hipMemsetAsync(c->grid, 0, sizeof(DevGrid), c->stream) == hipSuccess &&
               upload_tables(c) == LPS_OK;
     if (ok) {
@@ -1023,8 +1069,9 @@ int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const f
     c->have_step = false;
     c->epoch++;
     if ((st = upload_scenes(c))) return st;
-    lps_dev::launch_bbox(c->stream, c->pos[0], n, scenes(c), c->bbox_part);
+    lps_dev::launch_bbox(c->stream, c->pos[0], c->vel[0], c->der.mass, n, scenes(c), c->bbox_part);
     HIPC(hipStreamSynchronize(c->stream));
+    c->e_prev_valid = false;
     return LPS_OK;
 }
 
@@ -1120,8 +1167,9 @@ int lps_add_particles(lps_ctx* c, const float* x, const float* v, const float* v
     c->feat_valid = c->seed_valid = false;
     c->epoch++;
     if ((st = upload_scenes(c))) return st;
-    lps_dev::launch_bbox(s, c->pos[b], nslots, scenes(c), c->bbox_part);
+    lps_dev::launch_bbox(s, c->pos[b], c->vel[b], c->der.mass, nslots, scenes(c), c->bbox_part);
     HIPC(hipStreamSynchronize(s));
+    c->e_prev_valid = false;
     return LPS_OK;
 }
 
@@ -1253,12 +1301,15 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
         c->have_step = true;
         if (pci && last.ok) {
             // the correction loop ended normally, which the host has already
-            // seen; no synchronisation before the next step
+            // seen; no synchronisation before the next step.  The grid set-up
+            // of this step has written the energy entering it.
             completed += 1;
             retry_plain = false;
+            ws_energy(c, seeded);
             if (warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
             continue;
         }
+        c->e_prev_valid = false;
         HIPC(hipMemcpyAsync(c->h_grid, c->grid, sizeof(DevGrid), hipMemcpyDeviceToHost, c->stream));
         HIPC(hipStreamSynchronize(c->stream));
         completed = c->h_grid->steps_ok - base;
@@ -1295,6 +1346,25 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
         if (c->h_grid->steps_ok - base != nsteps) return LPS_ERR_HIP;
     }
     if (c->prof) prof_collect(c);
+    return LPS_OK;
+}
+
+int lps_get_energy(lps_ctx* c, double* kinetic, double* potential, uint32_t nscenes) {
+    if (!c || nscenes != c->h_sinfo.size() || (!kinetic && !potential)) return LPS_ERR_INVALID;
+    if (c->n_real == 0) return LPS_ERR_STATE;
+    HIPC(hipStreamSynchronize(c->stream));
+    // the current state's per-block records were written by the last
+    // integration or particle upload
+    std::vector<double> keep(c->h_energy, c->h_energy + 2 * (size_t)nscenes);
+    lps_dev::launch_scene_energy(c->stream, scenes(c), c->bbox_part, c->h_energy);
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(c->stream));
+    for (uint32_t k = 0; k < nscenes; k++) {
+        if (kinetic) kinetic[k] = c->h_energy[2 * k];
+        if (potential) potential[k] = c->h_energy[2 * k + 1];
+    }
+    // the fallback reads the energy entering the last step from the same array
+    std::copy(keep.begin(), keep.end(), c->h_energy);
     return LPS_OK;
 }
 

@@ -92,8 +92,9 @@ struct DevGrid {
     uint32_t list_of[LPS_MAX_ITER_SLOTS];
 };
 
-// Per-block partial bounding box: orderable-uint min xyz, max xyz, non-finite flag.
-#define LPS_BBOX_PART 8
+// Per-block partial record: orderable-uint min xyz, max xyz, non-finite flag,
+// then the block's kinetic and potential energy (float bits), one spare word.
+#define LPS_BBOX_PART 10
 
 // Per-step constants passed by value (the role of the 128-byte
 // ParallelSPHParameters block, ParallelSPHStructs.h:17-40).

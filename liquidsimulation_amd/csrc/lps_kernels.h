@@ -78,9 +78,17 @@ struct PhysArgs {
     SimConst c;
 };
 
-void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, uint32_t* part);
+// per-block partial records (bounding box and energies) of a state
+void launch_bbox(hipStream_t s, const float4* pos, const float4* vel, float mass, uint32_t n, const Scenes& sc,
+                 uint32_t* part);
 uint32_t bbox_parts(uint32_t n);
-void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap);
+// per-scene kinetic and potential energy from the partial records to
+// out[2 * scene], out[2 * scene + 1] (coherent host memory)
+void launch_scene_energy(hipStream_t s, const Scenes& sc, const uint32_t* part, double* out);
+// grid of the step; also writes each scene's energy of the entering state
+// to energy[2 * scene .. + 1] when energy is given
+void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap,
+                       double* energy);
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
                  uint32_t* vals,
                  uint32_t* tile_max);

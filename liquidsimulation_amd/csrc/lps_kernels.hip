@@ -45,6 +45,14 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
     for (int off = 32; off > 0; off >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, off, 64));
     return v;
 }
+__device__ __forceinline__ float wave_sum_f32(float v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 
 // 64-lane mask of the lanes holding the same 8-bit digit (8 ballots).
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
@@ -565,9 +573,11 @@ __device__ __forceinline__ void pair_dir(const float3& dl, float d, uint32_t i, 
 // Bounding box of the positions (SPHSolver.cpp:1011-1029).  Each block writes
 // its partial min/max (orderable-uint encoding) and a non-finite flag; the
 // grid set-up kernel reduces the partials, so there is no global atomic.
+// The same record carries the block's kinetic and potential energy (no
+// counterpart in the reference), summed per scene by the grid set-up.
 __device__ __forceinline__ void bbox_partial(uint32_t* __restrict__ part, uint32_t lb, bool valid, float x,
-                                             float y, float z) {
-    __shared__ uint32_t s_red[7][LPS_BLOCK / 64];
+                                             float y, float z, float ke, float pe) {
+    __shared__ uint32_t s_red[9][LPS_BLOCK / 64];
     uint32_t mn[3], mx[3], nf = 0;
     const float v[3] = {x, y, z};
     for (int a = 0; a < 3; a++) {
@@ -577,6 +587,7 @@ __device__ __forceinline__ void bbox_partial(uint32_t* __restrict__ part, uint32
         mx[a] = wave_max_u32(fin ? f2o(v[a]) : 0u);
     }
     nf = wave_max_u32(nf);
+    const float wke = wave_sum_f32(valid ? ke : 0.f), wpe = wave_sum_f32(valid ? pe : 0.f);
     const uint32_t w = threadIdx.x >> 6;
     if (lane_id() == 0) {
         for (int a = 0; a < 3; a++) {
@@ -584,6 +595,8 @@ __device__ __forceinline__ void bbox_partial(uint32_t* __restrict__ part, uint32
             s_red[3 + a][w] = mx[a];
         }
         s_red[6][w] = nf;
+        s_red[7][w] = __float_as_uint(wke);
+        s_red[8][w] = __float_as_uint(wpe);
     }
     __syncthreads();
     if (threadIdx.x < 7) {
@@ -591,26 +604,82 @@ __device__ __forceinline__ void bbox_partial(uint32_t* __restrict__ part, uint32
         uint32_t r = (a < 3) ? 0xFFFFFFFFu : 0u;
         for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) r = (a < 3) ? min(r, s_red[a][k]) : max(r, s_red[a][k]);
         part[lb * LPS_BBOX_PART + a] = r;
+    } else if (threadIdx.x < 9) {
+        float e = 0.f;
+        for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) e += __uint_as_float(s_red[threadIdx.x][k]);
+        part[lb * LPS_BBOX_PART + threadIdx.x] = __float_as_uint(e);
     }
 }
 
-__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, lps_dev::Scenes S,
-                                                    uint32_t* __restrict__ part, uint32_t nlogical) {
+// Kinetic and potential energy of one particle: m |v|^2 / 2 and -m g.x.
+__device__ __forceinline__ void particle_energy(float m, const float* g, float x, float y, float z, float vx,
+                                                float vy, float vz, float& ke, float& pe) {
+    ke = 0.5f * m * (vx * vx + vy * vy + vz * vz);
+    pe = -m * (g[0] * x + g[1] * y + g[2] * z);
+}
+
+__global__ __launch_bounds__(LPS_BLOCK) void k_bbox(const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                    float mass, lps_dev::Scenes S, uint32_t* __restrict__ part,
+                                                    uint32_t nlogical) {
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     const BlockScene B = block_scene(S, lb);
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     const bool valid = i < B.info.slot0 + B.info.count;
     float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (valid) p = pos[i];
-    bbox_partial(part, lb, valid, p.x, p.y, p.z);
+    float ke = 0.f, pe = 0.f;
+    if (valid) {
+        p = pos[i];
+        const float4 v = vel[i];
+        particle_energy(mass, B.info.gravity, p.x, p.y, p.z, v.x, v.y, v.z, ke, pe);
+    }
+    bbox_partial(part, lb, valid, p.x, p.y, p.z, ke, pe);
+}
+
+// Kinetic and potential energy of scene sidx from its blocks' partial records,
+// summed in double by one block, to out[2 * sidx] and out[2 * sidx + 1]
+// (coherent host memory).
+__device__ __forceinline__ void scene_energy_block(const lps_dev::Scenes& S, const uint32_t* __restrict__ part,
+                                                   uint32_t sidx, double* __restrict__ out) {
+    __shared__ double s_e[2][LPS_BLOCK / 64];
+    const SceneInfo si = S.info[sidx];
+    const uint32_t nparts = si.padded / LPS_BLOCK;
+    double ke = 0.0, pe = 0.0;
+    for (uint32_t k = threadIdx.x; k < nparts; k += LPS_BLOCK) {
+        const uint32_t* q = part + (si.block0 + k) * LPS_BBOX_PART;
+        ke += (double)__uint_as_float(q[7]);
+        pe += (double)__uint_as_float(q[8]);
+    }
+    ke = wave_sum_f64(ke);
+    pe = wave_sum_f64(pe);
+    if (lane_id() == 0) {
+        s_e[0][threadIdx.x >> 6] = ke;
+        s_e[1][threadIdx.x >> 6] = pe;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < LPS_BLOCK / 64; w++) {
+            ke += s_e[0][w];
+            pe += s_e[1][w];
+        }
+        out[2 * sidx] = ke;
+        out[2 * sidx + 1] = pe;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(LPS_BLOCK) void k_scene_energy(lps_dev::Scenes S, const uint32_t* __restrict__ part,
+                                                            double* __restrict__ out) {
+    scene_energy_block(S, part, blockIdx.x, out);
 }
 
 // Grid parameters of each scene (SPHSolver.cpp:1031-1040), one block per
 // scene over the partial boxes of its particle blocks.  min starts at FLT_MAX
 // and max at FLT_MIN as in the reference (SPHSolver.cpp:1013-1014).
 __device__ __forceinline__ void scene_grid_block(const lps_dev::Scenes& S, const uint32_t* __restrict__ part,
-                                                 float h, uint32_t sidx) {
+                                                 float h, uint32_t sidx, double* __restrict__ energy) {
+    // the energy of the state entering this step, for the host
+    if (energy) scene_energy_block(S, part, sidx, energy);
     __shared__ uint32_t s_r[7][LPS_BLOCK / 64];
     const SceneInfo si = S.info[sidx];
     const uint32_t nparts = si.padded / LPS_BLOCK;
@@ -661,8 +730,8 @@ __device__ __forceinline__ void scene_grid_block(const lps_dev::Scenes& S, const
 }
 
 __global__ __launch_bounds__(LPS_BLOCK) void k_scene_grid(lps_dev::Scenes S, const uint32_t* __restrict__ part,
-                                                          float h) {
-    scene_grid_block(S, part, h, blockIdx.x);
+                                                          float h, double* __restrict__ energy) {
+    scene_grid_block(S, part, h, blockIdx.x, energy);
 }
 
 // Cell numbering across scenes (scene s owns cells [cell_base, cell_base +
@@ -723,8 +792,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_finish(DevGrid* g, lps_dev::
 // scene's grid and then reads it (one launch less per step).
 __global__ __launch_bounds__(LPS_BLOCK) void k_grid_one(DevGrid* g, lps_dev::Scenes S,
                                                         const uint32_t* __restrict__ part, float h,
-                                                        uint32_t cell_cap) {
-    scene_grid_block(S, part, h, 0u);
+                                                        uint32_t cell_cap, double* __restrict__ energy) {
+    scene_grid_block(S, part, h, 0u, energy);
     __syncthreads();
     grid_finish_block(g, S, h, cell_cap);
 }
@@ -1501,6 +1570,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     const lps_sphere* spheres = spheres0 + B.info.sphere0;
     const lps_plane* planes = planes0 + B.info.plane0;
     float3 p = make_float3(0.f, 0.f, 0.f);
+    float ke = 0.f, pe = 0.f;
     if (valid) {
         const float4 x4 = pos[i];
         p = make_float3(x4.x, x4.y, x4.z);
@@ -1530,11 +1600,16 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
             collide_all(p, hv, c, boxes, spheres, planes);
             vh[i] = make_float4(hv.x, hv.y, hv.z, 0.f);
             pos[i] = make_float4(p.x, p.y, p.z, 0.f);
-            vel[i] = make_float4((old.x + hv.x) / 2.f, (old.y + hv.y) / 2.f, (old.z + hv.z) / 2.f, 0.f);
+            const float4 vn = make_float4((old.x + hv.x) / 2.f, (old.y + hv.y) / 2.f, (old.z + hv.z) / 2.f, 0.f);
+            vel[i] = vn;
+            particle_energy(c.mass, c.gravity, p.x, p.y, p.z, vn.x, vn.y, vn.z, ke, pe);
             if (pprev) pprev[i] = press[ob + i];
+        } else {
+            const float4 vo = vel[i];
+            particle_energy(c.mass, c.gravity, p.x, p.y, p.z, vo.x, vo.y, vo.z, ke, pe);
         }
     }
-    bbox_partial(bbox_part, lb, valid, p.x, p.y, p.z);
+    bbox_partial(bbox_part, lb, valid, p.x, p.y, p.z, ke, pe);
 }
 
 // Render interop: the real particles' positions, in readback order (scenes
@@ -1562,18 +1637,24 @@ namespace lps_dev {
 static inline uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1u) / b; }
 static inline uint32_t xcd_grid(uint32_t nlogical) { return cdiv(nlogical, LPS_XCDS) * LPS_XCDS; }
 
-void launch_bbox(hipStream_t s, const float4* pos, uint32_t n, const Scenes& sc, uint32_t* part) {
+void launch_bbox(hipStream_t s, const float4* pos, const float4* vel, float mass, uint32_t n, const Scenes& sc,
+                 uint32_t* part) {
     const uint32_t nl = cdiv(n, LPS_BLOCK);
-    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, sc, part, nl);
+    k_bbox<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(pos, vel, mass, sc, part, nl);
+}
+
+void launch_scene_energy(hipStream_t s, const Scenes& sc, const uint32_t* part, double* out) {
+    k_scene_energy<<<sc.count, LPS_BLOCK, 0, s>>>(sc, part, out);
 }
 
 uint32_t bbox_parts(uint32_t n) { return cdiv(n, LPS_BLOCK); }
 
-void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap) {
+void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap,
+                       double* energy) {
     if (sc.count == 1u) {
-        k_grid_one<<<1, LPS_BLOCK, 0, s>>>(g, sc, part, h, cell_cap);
+        k_grid_one<<<1, LPS_BLOCK, 0, s>>>(g, sc, part, h, cell_cap, energy);
     } else {
-        k_scene_grid<<<sc.count, LPS_BLOCK, 0, s>>>(sc, part, h);
+        k_scene_grid<<<sc.count, LPS_BLOCK, 0, s>>>(sc, part, h, energy);
         k_grid_finish<<<1, LPS_BLOCK, 0, s>>>(g, sc, h, cell_cap);
     }
 }

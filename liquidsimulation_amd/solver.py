@@ -277,6 +277,16 @@ class SPHSolver:
         check(lib().lps_get_fields(self._ctx, ptr(rho), ptr(p), ptr(f), ptr(rp)), "lps_get_fields")
         return {"rho": rho, "pressure": p, "force": f, "rho_pred": rp}
 
+    def get_energy(self):
+        """(kinetic, potential) energy of the current state, per scene
+        (float64 arrays), reduced on the device."""
+        nsc = C.c_uint32()
+        check(lib().lps_get_scene_count(self._ctx, C.byref(nsc)), "lps_get_scene_count")
+        ke = np.zeros(nsc.value, np.float64)
+        pe = np.zeros(nsc.value, np.float64)
+        check(lib().lps_get_energy(self._ctx, ke.ctypes.data, pe.ctypes.data, nsc.value), "lps_get_energy")
+        return ke, pe
+
     def delta(self, dt: float) -> float:
         d = C.c_float()
         check(lib().lps_get_delta(self._ctx, float(dt), C.byref(d)), "lps_get_delta")
