@@ -228,17 +228,38 @@ class EnsembleScene:
         return int(sum(m.n for m in self.members))
 
 
-def ensemble_64(seed: int = 4, count: int = 64, n_min: int = 8192, n_max: int = 131072) -> EnsembleScene:
-    """Batched ensemble: `count` dam breaks with particle counts log-uniform in
-    [n_min, n_max], drawn from splitmix64(seed): per-scene block dims (ny = 2 nx,
-    nz in [nx/2, 2 nx]), gravity tilted by up to +-10 degrees about z and x, one
-    obstacle sphere in the collapse path; spacing 0.00625 m and the 524k scene's
-    fluid and solver parameters, dt 0.001 s, 500 steps."""
+def _log_uniform_counts(u, n_min, n_max, total):
+    """Target counts n_min * (n_max / n_min)^(c u_k): log-uniform in
+    [n_min, n_min (n_max / n_min)^c], with c <= 1 found by bisection so that
+    the targets sum to `total` (c = 1 when the full range sums to less)."""
+    r = math.log(n_max / n_min)
+    f = lambda c: sum(n_min * math.exp(r * c * x) for x in u)
+    if f(1.0) <= total:
+        return 1.0
+    lo, hi = 0.0, 1.0
+    for _ in range(60):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if f(mid) < total else (lo, mid)
+    return 0.5 * (lo + hi)
+
+
+def ensemble_64(seed: int = 4, count: int = 64, n_min: int = 8192, n_max: int = 131072,
+                total: int = 2_000_000) -> EnsembleScene:
+    """Batched ensemble: `count` dam breaks with particle counts log-uniform
+    between n_min and at most n_max, about `total` in all, drawn from
+    splitmix64(seed).  A log-uniform draw over the whole of [8k, 131k] has a
+    mean of 44k per scene (2.8M for 64), so the range's upper end is narrowed
+    by a common exponent c (see _log_uniform_counts) to make the total about
+    2M.  Per-scene block dims (ny = 2 nx, nz in [nx/2, 2 nx]), gravity tilted
+    by up to +-10 degrees about z and x, one obstacle sphere in the collapse
+    path; spacing 0.00625 m and the 524k scene's fluid and solver parameters,
+    dt 0.001 s, 500 steps."""
     s_ = 0.00625
     u = uniform01(seed, 8 * count).astype(np.float64).reshape(count, 8)
+    c = _log_uniform_counts(u[:, 0], n_min, n_max, total)
     members, gravity = [], []
     for k in range(count):
-        n_t = math.exp(math.log(n_min) + (math.log(n_max) - math.log(n_min)) * u[k, 0])
+        n_t = math.exp(math.log(n_min) + (math.log(n_max) - math.log(n_min)) * c * u[k, 0])
         rz = 0.5 + 1.5 * u[k, 1]                       # nz / nx
         nx = max(4, int(round((n_t / (2.0 * rz)) ** (1.0 / 3.0))))
         ny, nz = 2 * nx, max(4, int(round(rz * nx)))
@@ -253,7 +274,7 @@ def ensemble_64(seed: int = 4, count: int = 64, n_min: int = 8192, n_max: int = 
         members.append(sc)
         gravity.append(tuple(float(v) for v in g))
     return EnsembleScene(name="ensemble_64", params=members[0].params, members=members, gravity=gravity,
-                         dt=0.001, steps=500, meta={"seed": seed, "count": count})
+                         dt=0.001, steps=500, meta={"seed": seed, "count": count, "range_exponent": c})
 
 
 WORKLOADS = {

@@ -100,3 +100,78 @@ def test_ensemble_grid_is_global(gpu):
     assert np.all(np.diff(keys[slices[0]].astype(np.int64)) >= 0)
     assert keys[slices[1]].min() > keys[slices[0]].max()
     e.close()
+
+
+@pytest.mark.parametrize("warm", ["none", "learned"])
+def test_ensemble_64_workload(gpu, oracle, warm):
+    """The configured ensemble: 64 dam breaks of 8k to 74k particles (about 2M
+    in all), each with its own block, gravity tilt, box and obstacle sphere,
+    stepped together for 3 steps (plain, and with the committed warm-start
+    network forced on).  Every scene equals the same scene run alone
+    (positions, velocities, per-scene corrections), and four of them (the
+    smallest, the largest and two others) equal the oracle's steps bit for
+    bit; with the seed on, the oracle gets the network's seed and the
+    previous step's pressure and applies the same envelope."""
+    from liquidsimulation_amd import warm_start
+    from liquidsimulation_amd.abi import WARM_KAPPA, WARM_LEARNED_ALWAYS, WARM_NONE
+    ens = scenes.ensemble_64()
+    assert 1_900_000 <= ens.n <= 2_100_000 and len(ens.members) == 64
+    mode = WARM_LEARNED_ALWAYS if warm == "learned" else WARM_NONE
+    steps = 3
+    e = Ensemble(ens.params.replace(warm_start=mode))
+    if warm == "learned":
+        warm_start.apply(e)
+    e.set_scenes([m.x for m in ens.members], [m.v for m in ens.members], gravity=ens.gravity)
+    for k, m in enumerate(ens.members):
+        e.set_scene_colliders(k, m.colliders)
+    e_its = []
+    for _ in range(steps):
+        st = e.update(ens.dt, 1)
+        assert st[0].status == 0
+        e_its.append([x.iterations for x in e.scene_stats()])
+    pe = e.get_particles()
+    slices = e.scene_slices()
+    e.close()
+    print(f"{ens.n} particles; corrections per scene, last step: {e_its[-1]}", flush=True)
+    for k, m in enumerate(ens.members):
+        s = solver_for(m.params.replace(gravity=ens.gravity[k], warm_start=mode))
+        if warm == "learned":
+            warm_start.apply(s)
+        s.set_colliders(m.colliders)
+        s.set_particles(m.x, m.v)
+        its = [s.update(m.dt, 1)[0].iterations for _ in range(steps)]
+        pa = s.get_particles()
+        s.close()
+        sl = slices[k]
+        assert [row[k] for row in e_its] == its, k
+        for key in ("x", "v", "vhalf"):
+            assert ulp_diff(pe[key][sl], pa[key]) == 0, (k, key)
+    ns = [m.n for m in ens.members]
+    sample = sorted({int(np.argmin(ns)), int(np.argmax(ns)), 17, 42})
+    for k in sample:
+        m = ens.members[k]
+        p = m.params.replace(gravity=ens.gravity[k])
+        if warm == "learned":
+            # the seeded steps need the network's own seed: run the scene
+            # alone on the device step by step and hand its seed to the oracle
+            s = solver_for(p.replace(warm_start=mode))
+            warm_start.apply(s)
+            s.set_colliders(m.colliders)
+            s.set_particles(m.x, m.v)
+        st = oracle.State.from_scene(m.x, m.v)
+        pprev = np.zeros(m.n, np.float32)
+        for step in range(steps):
+            if warm == "learned":
+                s.update(m.dt, 1)
+                _, _, seed = s.dump_pairs(seed=True)
+                tr = oracle.step_ocl_warm(p, m.colliders, m.dt, st, pprev, WARM_KAPPA, seed=seed)
+                pprev = tr.pressure.copy()
+            else:
+                tr = oracle.step_ocl(p, m.colliders, m.dt, st)
+            assert tr.iterations == e_its[step][k], (k, step)
+        if warm == "learned":
+            s.close()
+        sl = slices[k]
+        print(f"scene {k}: n={m.n} oracle corrections {tr.iterations}", flush=True)
+        for key, ref in (("x", st.x), ("v", st.v), ("vhalf", st.vh)):
+            assert ulp_diff(pe[key][sl], ref) == 0, (k, key)
