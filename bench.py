@@ -126,7 +126,7 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(
         if warmup:
             s.update(scene.dt, warmup)
         if tuning:
-            s.set_tuning(tuning["use_graphs"], tuning["lookahead"])
+            s.set_tuning(tuning["use_graphs"], tuning["lookahead"], tuning["correction_threads"])
         ke, pe = s.get_energy()
         e_samples.append(float(ke.sum() + pe.sum()))
         s.profile(True)

@@ -180,16 +180,21 @@ int lps_get_scene_count(lps_ctx* ctx, uint32_t* nscenes);
 /* Per-scene statistics of the last step (iterations, final error ratio, converged). */
 int lps_get_scene_stats(lps_ctx* ctx, lps_step_stats* stats /* [nscenes] */, uint32_t nscenes);
 
-/* Launch tuning (no counterpart in the reference).  use_graphs: replay the
- * fixed part of a PCISPH step (grid build, density, forces, corrections
- * 1..min) as a captured HIP graph; lookahead: corrections kept queued beyond
- * the one whose convergence result the host waits for.  lps_autotune times
- * `steps` steps of every candidate from the current state, restores the state
- * (the run continues exactly as without tuning) and keeps the fastest. */
+/* Launch tuning (no counterpart in the reference, which fixes its OpenCL
+ * workgroup size, SPHSolver.cpp:25).  use_graphs: replay the fixed part of a
+ * PCISPH step (grid build, density, forces, corrections 1..min) as a captured
+ * HIP graph; lookahead: corrections kept queued beyond the one whose
+ * convergence result the host waits for; correction_threads: workgroup size
+ * of the two correction kernels (256 runs a 256-slot particle block per
+ * workgroup, 128 and 64 split it over 2 or 4).  lps_autotune times `steps`
+ * steps of every combination from the current state, restores the state (the
+ * run continues exactly as without tuning) and keeps the fastest; every
+ * setting gives bit-identical results. */
 typedef struct lps_tuning {
-    int32_t use_graphs;   /* 0 or 1 */
-    uint32_t lookahead;   /* 1..4 */
-    float ms_per_step;    /* lps_autotune: measured time of the chosen setting */
+    int32_t use_graphs;            /* 0 or 1 */
+    uint32_t lookahead;            /* 1..4 */
+    float ms_per_step;             /* lps_autotune: measured time of the chosen setting */
+    uint32_t correction_threads;   /* 64, 128 or 256 */
 } lps_tuning;
 int lps_set_tuning(lps_ctx* ctx, const lps_tuning* tuning);
 int lps_get_tuning(lps_ctx* ctx, lps_tuning* tuning);

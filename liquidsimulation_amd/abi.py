@@ -109,6 +109,7 @@ class CTuning(C.Structure):
         ("use_graphs", C.c_int32),
         ("lookahead", C.c_uint32),
         ("ms_per_step", C.c_float),
+        ("correction_threads", C.c_uint32),
     ]
 
 

@@ -168,6 +168,7 @@ struct lps_ctx {
     uint64_t epoch = 1;
     bool graphs_on = true;
     uint32_t lookahead = 1;   // corrections kept queued ahead of the one waited for
+    uint32_t corr_threads = LPS_BLOCK;  // workgroup size of the correction kernels
 
     // profiling
     bool prof = false;
@@ -604,7 +605,8 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     if (mark) flag_slot(c, kk)[0] = FLAG_PENDING;
     {
         Timed t(c, "pci_density");
-        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, feat);
+        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, feat,
+                                    c->corr_threads);
     }
     if (kk == 1u && seeded) {
         // the first correction's pressure update is replaced by the seed
@@ -617,7 +619,7 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     {
         Timed t(c, "pci_pressure");
         lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp, c->xpred[kk & 1u],
-                                     kk, rule, c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS));
+                                     kk, rule, c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS), c->corr_threads);
     }
 }
 
@@ -1368,11 +1370,20 @@ int lps_get_energy(lps_ctx* c, double* kinetic, double* potential, uint32_t nsce
     return LPS_OK;
 }
 
+bool valid_corr_threads(uint32_t t) { return t == 64u || t == 128u || t == (uint32_t)LPS_BLOCK; }
+
+void set_corr_threads(lps_ctx* c, uint32_t t) {
+    if (c->corr_threads != t) c->epoch++;  // captured prefixes hold the old launch shape
+    c->corr_threads = t;
+}
+
 int lps_set_tuning(lps_ctx* c, const lps_tuning* t) {
-    if (!c || !t || (t->use_graphs != 0 && t->use_graphs != 1) || t->lookahead < 1 || t->lookahead > 4)
+    if (!c || !t || (t->use_graphs != 0 && t->use_graphs != 1) || t->lookahead < 1 || t->lookahead > 4 ||
+        !valid_corr_threads(t->correction_threads))
         return LPS_ERR_INVALID;
     c->graphs_on = t->use_graphs != 0;
     c->lookahead = t->lookahead;
+    set_corr_threads(c, t->correction_threads);
     return LPS_OK;
 }
 
@@ -1381,6 +1392,7 @@ int lps_get_tuning(lps_ctx* c, lps_tuning* t) {
     t->use_graphs = c->graphs_on ? 1 : 0;
     t->lookahead = c->lookahead;
     t->ms_per_step = 0.f;
+    t->correction_threads = c->corr_threads;
     return LPS_OK;
 }
 
@@ -1423,27 +1435,33 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
         return e;
     };
     const bool g0 = c->graphs_on;
-    const uint32_t l0 = c->lookahead;
+    const uint32_t l0 = c->lookahead, t0 = c->corr_threads;
     const uint32_t coll0 = c->coll_stride;
     c->coll_stride = 0;  // no training rows from trial steps
-    const double ema_s = c->ema_seeded, ema_p = c->ema_plain;
+    const double ema_s = c->ema_seeded, ema_p = c->ema_plain, ema_de = c->ema_de_plain;
     const bool ws = c->ws_on;
     const uint64_t wss = c->ws_steps, steps0 = c->steps;
     if (copy_all(true) != hipSuccess) st = LPS_ERR_HIP;
-    struct Cand { bool g; uint32_t l; float ms; };
+    // graphs x lookahead x correction-kernel workgroup size (the particle
+    // block of 256 slots run as 1, 2 or 4 workgroups)
+    struct Cand { bool g; uint32_t l, t; float ms; };
     std::vector<Cand> cands;
     for (int g = 0; g < 2; g++)
-        for (uint32_t l = 1; l <= 3; l++) cands.push_back({g == 1, l, 0.f});
+        for (uint32_t l = 1; l <= 3; l++)
+            for (uint32_t t : {(uint32_t)LPS_BLOCK, 128u, 64u}) cands.push_back({g == 1, l, t, 0.f});
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = LPS_ERR_HIP;
     for (Cand& cd : cands) {
         if (st) break;
         c->graphs_on = cd.g;
         c->lookahead = cd.l;
+        set_corr_threads(c, cd.t);
         // one untimed step (graph capture), then the timed ones, each from the snapshot
         for (int pass = 0; pass < 2 && !st; pass++) {
             c->cur = (int)cur;
             c->ema_seeded = ema_s; c->ema_plain = ema_p; c->ws_on = ws; c->ws_steps = wss;
+            c->ema_de_plain = ema_de;
+            c->e_prev_valid = false;
             if (copy_all(false) != hipSuccess) { st = LPS_ERR_HIP; break; }
             if (pass == 1) (void)hipEventRecord(e0, s);
             st = lps_step(c, dt, pass == 0 ? 1u : steps, nullptr);
@@ -1460,6 +1478,8 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
     // restore the state and the scheduling bookkeeping; keep the fastest
     c->cur = (int)cur;
     c->ema_seeded = ema_s; c->ema_plain = ema_p; c->ws_on = ws; c->ws_steps = wss; c->steps = steps0;
+    c->ema_de_plain = ema_de;
+    c->e_prev_valid = false;
     c->coll_stride = coll0;
     if (copy_all(false) != hipSuccess && !st) st = LPS_ERR_HIP;
     c->have_step = false;
@@ -1469,6 +1489,7 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
     if (st) {
         c->graphs_on = g0;
         c->lookahead = l0;
+        set_corr_threads(c, t0);
         return st;
     }
     const Cand* best = &cands[0];
@@ -1476,10 +1497,12 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
         if (cd.ms < best->ms) best = &cd;
     c->graphs_on = best->g;
     c->lookahead = best->l;
+    set_corr_threads(c, best->t);
     if (chosen) {
         chosen->use_graphs = best->g ? 1 : 0;
         chosen->lookahead = best->l;
         chosen->ms_per_step = best->ms;
+        chosen->correction_threads = best->t;
     }
     return LPS_OK;
 }

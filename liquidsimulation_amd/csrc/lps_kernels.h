@@ -105,8 +105,9 @@ void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
 // rho_pred and press: three buffers of a.mk.stride slots each; correction k
 // writes the scene's buffer buf_of[k % 64] (and reads the previous pressure
 // from buf_of[(k - 1) % 64]); q is a single buffer
+// threads: workgroup size of the correction kernels (64, 128 or LPS_BLOCK)
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, float* feat);
+                        float* q, uint32_t k, float* feat, uint32_t threads);
 
 // ---- learned warm start (lps_mlp.hip)
 // Network parameters, fp32 master copy in this order: W1 [64][16], b1 [64],
@@ -154,7 +155,7 @@ void launch_collect(hipStream_t s, const float* feat, uint32_t fs, const float* 
 // also applies the loop rule of correction k (block 0) and answers host_flag
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
                          const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
-                         uint32_t* host_flag);
+                         uint32_t* host_flag, uint32_t threads);
 // PCISPH (fp given): fp and press hold three buffers of a.mk.stride slots;
 // each scene reads the one its loop applied (SceneGrid::final_buf).  SPH:
 // press is the density pass's pressure.  pprev receives the applied pressure.

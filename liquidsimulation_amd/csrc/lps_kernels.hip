@@ -111,16 +111,26 @@ __device__ __forceinline__ BlockScene block_scene(const lps_dev::Scenes& S, uint
     return b;
 }
 
-// Particle block of this workgroup among the active blocks of list L, with
-// the active range split XCD-contiguously over the 8 XCDs; UINT_MAX when the
-// workgroup has no block.
-__device__ __forceinline__ uint32_t active_block(const DevGrid* g, const lps_dev::Scenes& S, uint32_t L) {
-    const uint32_t act = g->act_count[L];
+// The correction kernels may run a particle block of LPS_BLOCK slots as
+// LPS_BLOCK / WG workgroups of WG threads (the autotuner's choice): fewer
+// threads per workgroup spread the late corrections of an ensemble, when few
+// scenes still run, over more compute units.  Pieces of the active blocks of
+// list L are split XCD-contiguously over the 8 XCDs, so the pieces of one
+// block share an L2.  Returns the particle block and, in first_slot, the
+// workgroup's first slot; UINT_MAX when the workgroup has no piece.
+template <int WG>
+__device__ __forceinline__ uint32_t active_piece(const DevGrid* g, const lps_dev::Scenes& S, uint32_t L,
+                                                 uint32_t& first_slot) {
+    constexpr uint32_t P = LPS_BLOCK / WG;
+    const uint32_t act = g->act_count[L] * P;
     const uint32_t per = (act + LPS_XCDS - 1) / LPS_XCDS;
     const uint32_t b = blockIdx.x;
     if (b / LPS_XCDS >= per) return UINT_MAX;
-    const uint32_t lb = (b % LPS_XCDS) * per + b / LPS_XCDS;
-    return lb < act ? S.act_list[L * S.nblocks + lb] : UINT_MAX;
+    const uint32_t lp = (b % LPS_XCDS) * per + b / LPS_XCDS;
+    if (lp >= act) return UINT_MAX;
+    const uint32_t lb = S.act_list[L * S.nblocks + lp / P];
+    first_slot = lb * LPS_BLOCK + (lp % P) * (uint32_t)WG;
+    return lb;
 }
 
 // The solver constants as seen by one scene: its gravity and collider counts.
@@ -1262,8 +1272,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
 // from the current position, distances from the predicted ones.  Each block
 // adds its maximum to the scene's error of correction k; block 0 of the
 // pressure pass that follows applies the loop rule (loop_rule_block).
-template <bool FEAT>
-__global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
+template <bool FEAT, int WG>
+__global__ __launch_bounds__(WG) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
     float* __restrict__ q, uint32_t k, float* __restrict__ feat, const float4* __restrict__ posrho,
@@ -1273,16 +1283,17 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     if (step_dead(g) || g->running == 0u) return;
     // only blocks of scenes whose loop is running are scheduled
     const uint32_t L = g->list_of[k % LPS_MAX_ITER_SLOTS];
-    const uint32_t lb = active_block(g, S, L);
+    uint32_t s0 = 0u;
+    const uint32_t lb = active_piece<WG>(g, S, L, s0);
     if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    __shared__ uint32_t s_max[LPS_BLOCK / 64];
+    __shared__ uint32_t s_max[WG / 64];
     const BlockScene B = block_scene(S, lb);
     const bool active = true;
     load_table(sW, tables);
     load_thresholds(sT, tables);
     __syncthreads();
-    const uint32_t i = active ? lb * LPS_BLOCK + threadIdx.x : UINT_MAX;
+    const uint32_t i = active ? s0 + threadIdx.x : UINT_MAX;
     uint32_t ebits = 0;
     if (active && i < B.info.slot0 + B.info.count) {
         const GridView G = grid_view(g, S.grid + B.s);
@@ -1332,7 +1343,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density(
     // back and invalidates this XCD's L2 under every block still running)
     if (threadIdx.x == 0) {
         uint32_t m = 0;
-        for (uint32_t w = 0; w < LPS_BLOCK / 64; w++) m = max(m, s_max[w]);
+        for (uint32_t w = 0; w < WG / 64; w++) m = max(m, s_max[w]);
         if (active && m) atomicMax(&S.grid[B.s].err_bits[slot], m);
     }
 }
@@ -1376,7 +1387,7 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
     }
     __syncthreads();
     // one thread per scene
-    for (uint32_t sidx = threadIdx.x; sidx < S.count; sidx += LPS_BLOCK) {
+    for (uint32_t sidx = threadIdx.x; sidx < S.count; sidx += blockDim.x) {
         SceneGrid* sg = S.grid + sidx;
         if (sg->stop_iter != 0u) {
             atomicMax(&s_err, sg->final_bits);
@@ -1450,7 +1461,7 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
             if (S.grid[sidx].stop_iter != 0u) continue;
             const SceneInfo& si = S.info[sidx];
             const uint32_t nb = si.padded / LPS_BLOCK, at = s_nb;
-            for (uint32_t b = threadIdx.x; b < nb; b += LPS_BLOCK) S.act_list[N * S.nblocks + at + b] = si.block0 + b;
+            for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) S.act_list[N * S.nblocks + at + b] = si.block0 + b;
             __syncthreads();
             if (threadIdx.x == 0) s_nb = at + nb;
             __syncthreads();
@@ -1481,7 +1492,8 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
 
 // pciCalcPressureForce (SPHKernels.cl:806-878) followed by the next
 // prediction (pciIntegrate + pciHandleCollisions) from f + fp.
-__global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
+template <int WG>
+__global__ __launch_bounds__(WG) void k_pci_pressure(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, const float* __restrict__ rho_pred,
     const float* __restrict__ q, DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
@@ -1492,7 +1504,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     if (blockIdx.x == 0) loop_rule_block(g, S, k, rule, c0.rest_density, host_flag);
     if (step_dead(g)) return;
     // the blocks of the scenes that ran this correction's predicted-density pass
-    const uint32_t lb = active_block(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS]);
+    uint32_t s0 = 0u;
+    const uint32_t lb = active_piece<WG>(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS], s0);
     if (lb == UINT_MAX) return;
     const BlockScene B = block_scene(S, lb);
     const uint32_t stop = S.grid[B.s].stop_iter;
@@ -1501,7 +1514,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure(
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
     load_thresholds(sT, tables);
     __syncthreads();
-    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    const uint32_t i = s0 + threadIdx.x;
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
     const SimConst c = scene_const(c0, B.info);
@@ -1709,21 +1722,38 @@ void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
                                                      nl);
 }
 
-void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, float* feat) {
+template <int WG>
+static void pci_density_wg(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
+                           float* q, uint32_t k, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    auto kern = (feat && k == 1u) ? k_pci_density<true> : k_pci_density<false>;
-    kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press, q, k, feat,
-                                            a.posrho, a.sc, a.mk, nl);
+    auto kern = (feat && k == 1u) ? k_pci_density<true, WG> : k_pci_density<false, WG>;
+    kern<<<xcd_grid(nl * (LPS_BLOCK / WG)), WG, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press,
+                                                       q, k, feat, a.posrho, a.sc, a.mk, nl);
+}
+
+void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
+                        float* q, uint32_t k, float* feat, uint32_t threads) {
+    if (threads == 64u) pci_density_wg<64>(s, a, xpred, rho_pred, press, q, k, feat);
+    else if (threads == 128u) pci_density_wg<128>(s, a, xpred, rho_pred, press, q, k, feat);
+    else pci_density_wg<LPS_BLOCK>(s, a, xpred, rho_pred, press, q, k, feat);
+}
+
+template <int WG>
+static void pci_pressure_wg(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
+                            const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
+                            uint32_t* host_flag) {
+    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    k_pci_pressure<WG><<<xcd_grid(nl * (LPS_BLOCK / WG)), WG, 0, s>>>(
+        a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp, a.force, a.posrho, a.vel, a.vh, a.first,
+        a.boxes, a.spheres, a.planes, xpred_next, k, a.sc, a.mk, rule, host_flag, nl);
 }
 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
                          const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
-                         uint32_t* host_flag) {
-    const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
-    k_pci_pressure<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp,
-                                                      a.force, a.posrho, a.vel, a.vh, a.first, a.boxes, a.spheres,
-                                                      a.planes, xpred_next, k, a.sc, a.mk, rule, host_flag, nl);
+                         uint32_t* host_flag, uint32_t threads) {
+    if (threads == 64u) pci_pressure_wg<64>(s, a, xpred, rho_pred, q, fp, xpred_next, k, rule, host_flag);
+    else if (threads == 128u) pci_pressure_wg<128>(s, a, xpred, rho_pred, q, fp, xpred_next, k, rule, host_flag);
+    else pci_pressure_wg<LPS_BLOCK>(s, a, xpred, rho_pred, q, fp, xpred_next, k, rule, host_flag);
 }
 
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,

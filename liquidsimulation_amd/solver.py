@@ -293,14 +293,16 @@ class SPHSolver:
         return d.value
 
     # ---- launch tuning
-    def set_tuning(self, use_graphs: bool, lookahead: int = 1):
-        t = CTuning(use_graphs=int(bool(use_graphs)), lookahead=int(lookahead), ms_per_step=0.0)
+    def set_tuning(self, use_graphs: bool, lookahead: int = 1, correction_threads: int = 256):
+        t = CTuning(use_graphs=int(bool(use_graphs)), lookahead=int(lookahead), ms_per_step=0.0,
+                    correction_threads=int(correction_threads))
         check(lib().lps_set_tuning(self._ctx, C.byref(t)), "lps_set_tuning")
 
     def get_tuning(self):
         t = CTuning()
         check(lib().lps_get_tuning(self._ctx, C.byref(t)), "lps_get_tuning")
-        return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead}
+        return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead,
+                "correction_threads": t.correction_threads}
 
     def autotune(self, dt: float, steps: int = 10):
         """Time every launch setting for `steps` steps from the current state,
@@ -308,7 +310,8 @@ class SPHSolver:
         self._flush_pending()
         t = CTuning()
         check(lib().lps_autotune(self._ctx, float(dt), int(steps), C.byref(t)), "lps_autotune")
-        return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead, "ms_per_step": t.ms_per_step}
+        return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead, "ms_per_step": t.ms_per_step,
+                "correction_threads": t.correction_threads}
 
     # ---- timing
     def profile(self, on: bool = True):

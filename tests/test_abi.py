@@ -27,6 +27,7 @@ int main(void) {
   printf("lps_plane %zu\n", sizeof(lps_plane));
   printf("lps_step_stats %zu\n", sizeof(lps_step_stats));
   printf("lps_grid_info %zu\n", sizeof(lps_grid_info));
+  printf("lps_tuning %zu\n", sizeof(lps_tuning));
   printf("mlp_params %d\n", LPS_MLP_PARAMS);
   printf("abi %u\n", LPS_ABI_VERSION);
   printf("noalloc_rows %d\n", LPS_NOALLOC_MAX_ROWS);
@@ -71,6 +72,8 @@ def test_struct_layouts_match_header(c_layout):
     assert c_layout["lps_sphere"] == C.sizeof(CSphere)
     assert c_layout["lps_plane"] == C.sizeof(CPlane)
     assert c_layout["lps_step_stats"] == C.sizeof(CStepStats)
+    from liquidsimulation_amd.abi import CTuning
+    assert c_layout["lps_tuning"] == C.sizeof(CTuning)
     assert c_layout["lps_grid_info"] == C.sizeof(CGridInfo)
     assert c_layout["mlp_params"] == LPS_MLP_PARAMS
     assert c_layout["abi"] == LPS_ABI_VERSION

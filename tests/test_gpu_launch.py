@@ -1,6 +1,7 @@
 """GPU tests of the launch machinery: the captured-graph step prefix, the
-correction lookahead and the autotuner change nothing but timing -- states,
-iteration counts and seeds are bit-identical to the eager path."""
+correction lookahead, the correction kernels' workgroup size and the
+autotuner change nothing but timing -- states, iteration counts and seeds are
+bit-identical to the eager path."""
 import numpy as np
 import pytest
 
@@ -19,13 +20,13 @@ def _scene():
     return sc
 
 
-def _run(sc, steps, use_graphs, lookahead, warm=None, autotune=False, one_call=False):
+def _run(sc, steps, use_graphs, lookahead, warm=None, autotune=False, one_call=False, threads=256):
     s = PCISPHSolver(sc.params.replace(warm_start=WARM_LEARNED) if warm is not None else sc.params)
     if warm is not None:
         s.set_warm_start_weights(*warm)
     s.set_colliders(sc.colliders)
     s.set_particles(sc.x, sc.v)
-    s.set_tuning(use_graphs, lookahead)
+    s.set_tuning(use_graphs, lookahead, threads)
     tuned = s.autotune(sc.dt, 3) if autotune else None
     its, used = [], []
     for st in (s.update(sc.dt, steps) if one_call else [s.update(sc.dt, 1)[0] for _ in range(steps)]):
@@ -47,11 +48,11 @@ def test_graph_prefix_and_lookahead_match_eager(gpu, seeded):
     ref, its_ref, used_ref, _ = _run(sc, 24, False, 1, warm)
     print("eager iterations:", its_ref, "seeded:", used_ref, flush=True)
     assert max(its_ref) > 3
-    for g, la in ((True, 1), (True, 2), (False, 3)):
-        out, its, used, _ = _run(sc, 24, g, la, warm)
-        assert its == its_ref and used == used_ref, (g, la)
+    for g, la, t in ((True, 1, 256), (True, 2, 128), (False, 3, 64), (False, 1, 128), (True, 1, 64)):
+        out, its, used, _ = _run(sc, 24, g, la, warm, threads=t)
+        assert its == its_ref and used == used_ref, (g, la, t)
         for k in ("x", "v", "vhalf"):
-            assert ulp_diff(out[k], ref[k]) == 0, (g, la, k)
+            assert ulp_diff(out[k], ref[k]) == 0, (g, la, t, k)
 
 
 def test_autotune_leaves_the_trajectory_unchanged(gpu):
