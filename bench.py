@@ -10,24 +10,28 @@ of the whole scene; the state stays resident in HBM during the timed region.
 Rate = particles * K / wall seconds of the K timed steps (HIP events on the
 library's stream, bracketed by device synchronisation).
 
-Both modes run the same K steps from the same initial state: plain (zero
-pressure start, as the reference) and learned (the committed network seeds
-the first correction; same loop, same tolerance, the library's fallback on).
-`value` is the learned mode's rate only when that run is at least as accurate
-as the plain one (no more steps missing the tolerance, no larger worst step
-error, no higher energy peak); otherwise it is the plain rate (`value_mode`,
-`value_reason` say which and why).  Neither mode meets 1 % in every step of
-this scene: the reference's update rule and clamping box collider leave
-particles pressed against the walls above 1 % from step 23 on, and the scene
-turns unstable in every mode once the spreading front reaches the far z wall
-(about step 233); each mode reports its qualified-step fraction, and the
-timed run is split into windows (steps 5-45, the CPU baseline's sample, and
-5-235, before the instability) whose rates and errors are reported too.
+Three modes run the same K steps from the same initial state: plain (zero
+pressure start, as the reference), learned (the committed network seeds the
+first correction; same loop, same tolerance, the library's fallback on) and
+learned_always (the seed on every step, no fallback).  A learned mode's rate
+counts only when its run is at least as accurate as the plain one (no more
+steps missing the tolerance, no larger worst step error, no higher energy
+peak); `value` is the fastest mode that counts (`value_mode`, `value_reason`
+say which and why).
+
+Without --steps the run times steps 5-235 of the scene's configured 2000: in
+every mode (and on the CPU oracle) the reference's update rule and clamping
+box collider leave particles pressed against the walls above 1 % from step
+23 on, and the scene turns unstable once the spreading front reaches the far
+z wall (about step 233), after which the run measures an exploding state.
+Each mode reports its qualified-step fraction; --steps 2000 times the whole
+configured run.
 
 Prints ONE JSON line.  Extra keys: per-step PCISPH iteration statistics,
 convergence qualification, the dominant kernel's roofline (algorithmic bytes
 per launch over its event-timed average duration) and the CPU baseline (the
-oracle's OpenMP restatement on this host, bounded sample).
+oracle's OpenMP restatement on this host, over the same steps as the GPU's
+timed run up to 40 of them, and the GPU's rate over exactly those steps).
 """
 from __future__ import annotations
 
@@ -82,7 +86,7 @@ def _solver(scene, weights):
 
 def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(40, 230)):
     """Timed run of `steps` steps after `warmup`, with HIP events on the
-    library's stream after the first marks[0] and marks[1] timed steps (no
+    library's stream after the first marks[0], marks[1], ... timed steps (no
     synchronisation between them), then (profile) the same warm-up + steps
     again from the same initial state with per-launch HIP events: the run is
     deterministic, so the per-kernel averages are those of the timed steps;
@@ -94,7 +98,7 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(
         s.update(scene.dt, warmup)
     tuning = s.autotune(scene.dt, 10) if scene.params.method == 1 else None
     stream = torch.cuda.ExternalStream(s.stream())
-    bounds = [0] + [m for m in marks if 0 < m < steps] + [steps]
+    bounds = [0] + sorted({m for m in marks if 0 < m < steps}) + [steps]
     evs = [torch.cuda.Event(enable_timing=True) for _ in bounds]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -230,7 +234,7 @@ def bench_ensemble(args):
     plain = run_ensemble(ens, steps, args.warmup, None, not args.no_profile)
     learned = run_ensemble(ens, steps, args.warmup, w, False) if w is not None else None
     alone = run_members_alone(ens, max(20, steps // 10), args.warmup)
-    value_mode, value_reason = choose_value(plain, learned)
+    value_mode, value_reason = choose_value(plain, {"learned": learned})
     head = learned if value_mode == "learned" else plain
     out = {
         "metric": "particle-steps/s of an ensemble of independent PCISPH dam breaks in one context, 1 GPU",
@@ -290,13 +294,7 @@ def mode_summary(res, scene, steps):
     }
 
 
-def choose_value(plain, learned):
-    """The learned rate counts only when the learned run is at least as
-    accurate as the plain one over the same steps: no more steps that miss
-    the tolerance, no larger worst step error, and no higher energy peak
-    (5 % slack on the energy).  Otherwise the plain rate is the value."""
-    if learned is None:
-        return "plain", "no weights"
+def _accuracy_reasons(plain, learned):
     reasons = []
     if learned["nonconverged_steps"] > plain["nonconverged_steps"]:
         reasons.append("more steps miss the tolerance")
@@ -305,7 +303,28 @@ def choose_value(plain, learned):
     el, ep = learned["energy_J"], plain["energy_J"]
     if el and ep and el["max"] > 1.05 * max(ep["max"], ep["start"]):
         reasons.append("higher energy peak")
-    return ("plain", "learned: " + ", ".join(reasons)) if reasons else ("learned", "learned at least as accurate")
+    return reasons
+
+
+def choose_value(plain, learned):
+    """A learned mode's rate counts only when its run is at least as accurate
+    as the plain one over the same steps: no more steps that miss the
+    tolerance, no larger worst step error, and no higher energy peak (5 %
+    slack on the energy).  `value` is the fastest of the modes that count;
+    plain always counts.  `learned` maps mode name -> summary (or None)."""
+    best, why = "plain", []
+    for name, lm in learned.items():
+        if lm is None:
+            continue
+        reasons = _accuracy_reasons(plain, lm)
+        if reasons:
+            why.append(f"{name}: " + ", ".join(reasons))
+        elif lm["value"] < (plain if best == "plain" else learned[best])["value"]:
+            why.append(f"{name}: at least as accurate, slower")
+        else:
+            why.append(f"{name}: at least as accurate, faster")
+            best = name
+    return best, "; ".join(why) if why else "no weights"
 
 
 def cpu_baseline(scene, warm: int, timed: int):
@@ -383,14 +402,15 @@ def measured_traffic(kernel: str):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--steps", type=int, default=230)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="pcisph_dam_break_524k")
     ap.add_argument("--no-profile", action="store_true",
                     help="skip the event-timed rerun that gives per-kernel averages")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-warm", type=int, default=5)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="timed CPU steps (default: the GPU's timed steps, at most 40)")
     ap.add_argument("--mode", choices=("both", "plain"), default="both")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy five times (scales the HBM byte counters)")
@@ -405,27 +425,32 @@ def main():
         sys.exit(2)
 
     from liquidsimulation_amd import scenes, warm_start
-    from liquidsimulation_amd.abi import WARM_LEARNED, WARM_NONE
+    from liquidsimulation_amd.abi import WARM_LEARNED, WARM_LEARNED_ALWAYS, WARM_NONE
     if args.workload.startswith("ensemble"):
         bench_ensemble(args)
         return
     scene = scenes.WORKLOADS[args.workload]()
-    learned = res_l = None
+    cpu_steps = args.cpu_steps if args.cpu_steps > 0 else min(args.steps, 40)
+    marks = (cpu_steps, 230)
+    summaries = {"learned": None, "learned_always": None}
+    runs = {}
     if args.mode == "both" and scene.params.method == 1 and os.path.exists(warm_start.WEIGHTS):
         w = warm_start.load()
-        sl = scene.with_params(warm_start=WARM_LEARNED)
-        res_l = run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w)
-        learned = mode_summary(res_l, sl, args.steps)
+        for name, mode in (("learned", WARM_LEARNED), ("learned_always", WARM_LEARNED_ALWAYS)):
+            sl = scene.with_params(warm_start=mode)
+            runs[name] = run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w, marks=marks)
+            summaries[name] = mode_summary(runs[name], sl, args.steps)
     scene = scene.with_params(warm_start=WARM_NONE)
-    res = run_gpu(scene, args.steps, args.warmup, not args.no_profile)
+    runs["plain"] = res = run_gpu(scene, args.steps, args.warmup, not args.no_profile, marks=marks)
     plain = mode_summary(res, scene, args.steps)
+    learned = summaries["learned"]
     n = scene.n
-    value_mode, value_reason = choose_value(plain, learned)
-    head = learned if value_mode == "learned" else plain
+    value_mode, value_reason = choose_value(plain, summaries)
+    head = plain if value_mode == "plain" else summaries[value_mode]
     value = head["value"]
     tol = scene.params.max_density_error_ratio
     # the roofline is that of the run `value` comes from
-    kernels = (res_l if value_mode == "learned" else res)["kernels"]
+    kernels = runs[value_mode]["kernels"]
     dom = max(kernels.items(), key=lambda kv: kv[1]["total_ms"])[0] if kernels else None
     roof = None
     if dom:
@@ -466,7 +491,10 @@ def main():
         "worst_density_error": head["worst_density_error"],
         "plain": plain,
         "learned": learned,
+        "learned_always": summaries["learned_always"],
         "speedup_learned_vs_plain": (learned["value"] / plain["value"]) if learned is not None else None,
+        "speedup_learned_always_vs_plain": (summaries["learned_always"]["value"] / plain["value"])
+        if summaries["learned_always"] is not None else None,
         "roofline": roof,
         "issue_roofline": issue_roofline(dom) if dom else None,
         "step_roofline": {"achieved_GBps": step_roof, "frac": step_roof / HBM_PEAK_GBS,
@@ -475,13 +503,19 @@ def main():
         "wall_s": res["wall"],
     }
     if not args.no_cpu:
-        cb = cpu_baseline(scene, args.cpu_warm, args.cpu_steps)
-        # the GPU over the same steps (the first window of the timed run)
-        win = f"steps {args.warmup}-{args.warmup + args.cpu_steps}"
-        if args.cpu_warm == args.warmup and win in head["windows"]:
-            cb["gpu_same_steps"] = {"window": win, "value": head["windows"][win]["value"],
-                                    "iterations_mean": head["windows"][win]["iterations_mean"],
-                                    "gpu_over_cpu": head["windows"][win]["value"] / cb["value"]}
+        cb = cpu_baseline(scene, args.cpu_warm, cpu_steps)
+        # the GPU over the same steps (the whole timed run, or its first window)
+        if args.cpu_warm == args.warmup:
+            if cpu_steps == args.steps:
+                same = {"window": f"steps {args.warmup}-{args.warmup + args.steps}", "value": head["value"],
+                        "iterations_mean": head["iterations"]["mean"]}
+            else:
+                win = f"steps {args.warmup}-{args.warmup + cpu_steps}"
+                same = dict(window=win, **{k: head["windows"][win][k] for k in ("value", "iterations_mean")}) \
+                    if win in head["windows"] else None
+            if same:
+                same["gpu_over_cpu"] = same["value"] / cb["value"]
+                cb["gpu_same_steps"] = same
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None

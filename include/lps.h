@@ -237,12 +237,16 @@ int lps_get_fields(lps_ctx* ctx, float* rho, float* pressure, float* force /* [3
  * particle's applied pressure of the previous step): an over-prediction
  * shrinks by LPS_WARM_KAPPA per step instead of feeding itself through
  * feature 7.  With negative pressures enabled it becomes the seed.
- * With warm_start = LPS_WARM_LEARNED and weights
- * loaded the seed is used unless the running mean of corrections per step
- * with it exceeds the one without it (every 32nd step runs the other mode to
- * keep both current); a seeded step that turns non-finite is re-run unseeded,
- * and one that adds mechanical energy (LPS_WARM_ENERGY_TOL) switches the seed
- * off until the next probe. */
+ * With warm_start = LPS_WARM_LEARNED and weights loaded the seed is judged on
+ * paired samples: wherever two consecutive steps ran in different modes, the
+ * seeded step's cost minus the zero start's, a step's cost being its
+ * corrections if it met the tolerance and max_iterations if it did not.
+ * Every 16th step runs the mode not in use (giving two samples), and the
+ * seed is used while the sum of the last 8 samples is not positive, i.e.
+ * while its iteration count to the tolerance is not worse.  A seeded step
+ * that turns non-finite is re-run unseeded, and it, like a seeded step that
+ * adds mechanical energy (LPS_WARM_ENERGY_TOL), enters as a sample of
+ * max_iterations. */
 /* Seed envelope factor: with negative pressures disabled the first
  * correction's pressure is max(p1, min(seed, LPS_WARM_KAPPA * p_prev)), p1
  * the zero start's own first-correction pressure and p_prev the particle's

@@ -68,6 +68,53 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+// The learned seed's fallback (LPS_WARM_LEARNED).  The corrections a step
+// needs vary several-fold from one step to the next and drift over a run, so
+// the seed is judged on paired samples: wherever two consecutive steps ran in
+// different modes, the seeded one's cost minus the zero start's.  A step's
+// cost is its corrections if it met the tolerance and max_iterations if it
+// did not (a loop the stall rule ended early has not reached the tolerance,
+// whatever its count).  The seed is used while the sum of the last WS_WINDOW
+// samples is not positive; every WS_PROBE_PERIOD-th step runs the other mode,
+// which gives two samples (the probe against the step before it and the step
+// after it).  A seeded step that fails (non-finite, or adding energy) enters
+// as a sample of max_iterations.
+constexpr uint64_t WS_PROBE_PERIOD = 16;
+constexpr uint32_t WS_WINDOW = 8;
+
+struct WarmFallback {
+    double win[WS_WINDOW] = {};
+    uint32_t n = 0, head = 0;
+    int32_t prev_its = 0;
+    int8_t prev_seeded = -1;     // mode of the last recorded step, -1 none
+    bool on = true;
+    uint64_t steps = 0;          // steps recorded since the weights or the state changed
+    double ema_de_plain = 0.0;   // running relative energy change of zero-start steps
+
+    void push(double d) {
+        win[head] = d;
+        head = (head + 1) % WS_WINDOW;
+        if (n < WS_WINDOW) n++;
+        double sum = 0.0;
+        for (uint32_t i = 0; i < n; i++) sum += win[i];
+        on = sum <= 0.0;
+    }
+    bool probe() const { return steps % WS_PROBE_PERIOD == WS_PROBE_PERIOD - 1u; }
+    // new weights: no samples, seed on (the zero start's energy mean stays)
+    void restart() {
+        const double de = ema_de_plain;
+        *this = WarmFallback{};
+        ema_de_plain = de;
+    }
+    void record(bool seeded, int32_t its) {
+        if (prev_seeded >= 0 && (prev_seeded != 0) != seeded)
+            push(seeded ? (double)(its - prev_its) : (double)(prev_its - its));
+        prev_seeded = seeded ? 1 : 0;
+        prev_its = its;
+        steps++;
+    }
+};
+
 }  // namespace
 
 struct lps_ctx {
@@ -129,7 +176,6 @@ struct lps_ctx {
     std::vector<double> e_prev;  // per scene energy entering the previous step
     bool e_prev_valid = false;
     bool e_prev_seeded = false;
-    double ema_de_plain = 0.0;   // running relative energy change of zero-start steps
     int last_fp_valid = 0;      // PCISPH fields available
 
     // learned warm start: fp32 master weights, feature normalisation, target scale
@@ -144,10 +190,7 @@ struct lps_ctx {
     float* seed = nullptr;        // [cap_n] seeded pressure of the current step
     bool feat_valid = false;      // the last step wrote features (and final pressure)
     bool seed_valid = false;      // the last step applied the seed
-    // fallback: exponential averages of corrections per step with and without the seed
-    double ema_seeded = -1.0, ema_plain = -1.0;
-    bool ws_on = true;
-    uint64_t ws_steps = 0;
+    WarmFallback wf;              // the seed's fallback rule (seed_this_step)
     // training-set collection and training
     float *coll_x = nullptr, *coll_t = nullptr;
     uint64_t coll_cap = 0, coll_rows = 0;
@@ -491,7 +534,6 @@ struct StepResult {
 };
 
 // ---- learned warm start: when to seed, fallback bookkeeping
-constexpr uint64_t WS_PROBE_PERIOD = 32;  // every 32nd step runs the other mode
 
 bool warm_ready(const lps_ctx* c) {
     return c->params.method == LPS_METHOD_PCISPH &&
@@ -499,24 +541,19 @@ bool warm_ready(const lps_ctx* c) {
            c->mlp_loaded;
 }
 
-// Seed unless the averaged corrections per step with the seed exceed those
-// without it; every WS_PROBE_PERIOD-th step runs the mode not in use so both
-// averages stay current.
+// The mode in use (WarmFallback::on), except on probe steps.
 bool seed_this_step(const lps_ctx* c) {
     if (!warm_ready(c)) return false;
     if (c->params.warm_start == LPS_WARM_LEARNED_ALWAYS) return true;
-    if (c->ema_seeded < 0.0) return true;   // first measurement with the seed,
-    if (c->ema_plain < 0.0) return false;   // then one without
-    const bool probe = (c->ws_steps % WS_PROBE_PERIOD) == WS_PROBE_PERIOD - 1u;
-    return probe ? !c->ws_on : c->ws_on;
+    return c->wf.probe() ? !c->wf.on : c->wf.on;
 }
 
-void ws_record(lps_ctx* c, bool seeded, double iterations) {
-    double& e = seeded ? c->ema_seeded : c->ema_plain;
-    e = e < 0.0 ? iterations : 0.7 * e + 0.3 * iterations;
-    c->ws_on = c->ema_plain < 0.0 || c->ema_seeded < 0.0 || c->ema_seeded <= c->ema_plain;
-    c->ws_steps++;
+void ws_record(lps_ctx* c, bool seeded, int32_t iterations, bool converged) {
+    c->wf.record(seeded, converged ? iterations : (int32_t)c->params.max_iterations);
 }
+
+// A seeded step that failed counts as max_iterations corrections worse.
+void ws_fail(lps_ctx* c) { c->wf.push((double)c->params.max_iterations); }
 
 // Energy judgement of the previous step, once the grid set-up of the current
 // one has written the energy entering it (h_energy).  With negative pressures
@@ -538,12 +575,9 @@ void ws_energy(lps_ctx* c, bool seeded_now) {
         }
         if (std::isfinite(worst)) {
             if (c->e_prev_seeded) {
-                if (worst > std::max(c->ema_de_plain, 0.0) + (double)LPS_WARM_ENERGY_TOL) {
-                    c->ema_seeded = (double)c->params.max_iterations + 1.0;
-                    c->ws_on = false;
-                }
+                if (worst > std::max(c->wf.ema_de_plain, 0.0) + (double)LPS_WARM_ENERGY_TOL) ws_fail(c);
             } else {
-                c->ema_de_plain = 0.7 * c->ema_de_plain + 0.3 * worst;
+                c->wf.ema_de_plain = 0.7 * c->wf.ema_de_plain + 0.3 * worst;
             }
         }
     }
@@ -1308,7 +1342,7 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
             completed += 1;
             retry_plain = false;
             ws_energy(c, seeded);
-            if (warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
+            if (warm_ready(c)) ws_record(c, seeded, last.iterations, last.converged != 0);
             continue;
         }
         c->e_prev_valid = false;
@@ -1327,8 +1361,7 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
             // the seeded step failed and left the state as it was: run it
             // again without the seed and count the failure against the seed
             ++retries;
-            c->ema_seeded = (double)c->params.max_iterations + 1.0;
-            c->ws_on = false;
+            ws_fail(c);
             retry_plain = true;
             continue;
         }
@@ -1338,7 +1371,7 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
             return (int)status;
         }
         if (pci && last.iterations < 0) return LPS_ERR_HIP;
-        if (pci && warm_ready(c)) ws_record(c, seeded, (double)last.iterations);
+        if (pci && warm_ready(c)) ws_record(c, seeded, last.iterations, last.converged != 0);
     }
     if (pci) {
         // one check per call: every step must have completed on the device
@@ -1438,9 +1471,8 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
     const uint32_t l0 = c->lookahead, t0 = c->corr_threads;
     const uint32_t coll0 = c->coll_stride;
     c->coll_stride = 0;  // no training rows from trial steps
-    const double ema_s = c->ema_seeded, ema_p = c->ema_plain, ema_de = c->ema_de_plain;
-    const bool ws = c->ws_on;
-    const uint64_t wss = c->ws_steps, steps0 = c->steps;
+    const WarmFallback wf0 = c->wf;
+    const uint64_t steps0 = c->steps;
     if (copy_all(true) != hipSuccess) st = LPS_ERR_HIP;
     // graphs x lookahead x correction-kernel workgroup size (the particle
     // block of 256 slots run as 1, 2 or 4 workgroups)
@@ -1459,8 +1491,7 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
         // one untimed step (graph capture), then the timed ones, each from the snapshot
         for (int pass = 0; pass < 2 && !st; pass++) {
             c->cur = (int)cur;
-            c->ema_seeded = ema_s; c->ema_plain = ema_p; c->ws_on = ws; c->ws_steps = wss;
-            c->ema_de_plain = ema_de;
+            c->wf = wf0;
             c->e_prev_valid = false;
             if (copy_all(false) != hipSuccess) { st = LPS_ERR_HIP; break; }
             if (pass == 1) (void)hipEventRecord(e0, s);
@@ -1477,8 +1508,8 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
     }
     // restore the state and the scheduling bookkeeping; keep the fastest
     c->cur = (int)cur;
-    c->ema_seeded = ema_s; c->ema_plain = ema_p; c->ws_on = ws; c->ws_steps = wss; c->steps = steps0;
-    c->ema_de_plain = ema_de;
+    c->wf = wf0;
+    c->steps = steps0;
     c->e_prev_valid = false;
     c->coll_stride = coll0;
     if (copy_all(false) != hipSuccess && !st) st = LPS_ERR_HIP;
@@ -1621,9 +1652,7 @@ int lps_mlp_set_weights(lps_ctx* c, const float* params, const float* feature_sc
     HIPC(hipStreamSynchronize(c->stream));
     c->mlp_loaded = true;
     c->epoch++;
-    c->ema_seeded = c->ema_plain = -1.0;
-    c->ws_on = true;
-    c->ws_steps = 0;
+    c->wf.restart();
     return LPS_OK;
 }
 
@@ -1814,9 +1843,7 @@ int lps_mlp_train(lps_ctx* c, const lps_mlp_train_cfg* cfg, float* epoch_loss) {
         }
     c->mlp_loaded = true;
     c->epoch++;
-    c->ema_seeded = c->ema_plain = -1.0;
-    c->ws_on = true;
-    c->ws_steps = 0;
+    c->wf.restart();
     return LPS_OK;
 }
 
