@@ -84,7 +84,7 @@ def _solver(scene, weights):
     return s
 
 
-def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(40, 230)):
+def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(40, 230), fixed_tuning=None):
     """Timed run of `steps` steps after `warmup`, with HIP events on the
     library's stream after the first marks[0], marks[1], ... timed steps (no
     synchronisation between them), then (profile) the same warm-up + steps
@@ -96,7 +96,11 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(
     s = _solver(scene, weights)
     if warmup:
         s.update(scene.dt, warmup)
-    tuning = s.autotune(scene.dt, 10) if scene.params.method == 1 else None
+    if fixed_tuning is not None:
+        s.set_tuning(*fixed_tuning)
+        tuning = dict(zip(("use_graphs", "lookahead", "correction_threads"), fixed_tuning), fixed=True)
+    else:
+        tuning = s.autotune(scene.dt, 10) if scene.params.method == 1 else None
     stream = torch.cuda.ExternalStream(s.stream())
     bounds = [0] + sorted({m for m in marks if 0 < m < steps}) + [steps]
     evs = [torch.cuda.Event(enable_timing=True) for _ in bounds]
@@ -412,6 +416,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="timed CPU steps (default: the GPU's timed steps, at most 40)")
     ap.add_argument("--mode", choices=("both", "plain"), default="both")
+    ap.add_argument("--tuning", default=None,
+                    help="graphs,lookahead,threads (e.g. 0,2,256): fixed launch settings, no autotuning "
+                         "(counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy five times (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -430,6 +437,10 @@ def main():
         bench_ensemble(args)
         return
     scene = scenes.WORKLOADS[args.workload]()
+    fixed = None
+    if args.tuning:
+        g_, l_, t_ = (int(v) for v in args.tuning.split(","))
+        fixed = (bool(g_), l_, t_)
     cpu_steps = args.cpu_steps if args.cpu_steps > 0 else min(args.steps, 40)
     marks = (cpu_steps, 230)
     summaries = {"learned": None, "learned_always": None}
@@ -438,10 +449,12 @@ def main():
         w = warm_start.load()
         for name, mode in (("learned", WARM_LEARNED), ("learned_always", WARM_LEARNED_ALWAYS)):
             sl = scene.with_params(warm_start=mode)
-            runs[name] = run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w, marks=marks)
+            runs[name] = run_gpu(sl, args.steps, args.warmup, not args.no_profile, weights=w, marks=marks,
+                                 fixed_tuning=fixed)
             summaries[name] = mode_summary(runs[name], sl, args.steps)
     scene = scene.with_params(warm_start=WARM_NONE)
-    runs["plain"] = res = run_gpu(scene, args.steps, args.warmup, not args.no_profile, marks=marks)
+    runs["plain"] = res = run_gpu(scene, args.steps, args.warmup, not args.no_profile, marks=marks,
+                                  fixed_tuning=fixed)
     plain = mode_summary(res, scene, args.steps)
     learned = summaries["learned"]
     n = scene.n

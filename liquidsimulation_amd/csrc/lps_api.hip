@@ -144,6 +144,7 @@ struct lps_ctx {
     uint32_t* digit_total = nullptr;
     uint32_t* bbox_part = nullptr;
     float4 *posrho = nullptr, *force = nullptr, *fp = nullptr, *xpred[2] = {};
+    float4* xq = nullptr;  // predicted-density pass: (prediction, p / rho*^2) [cap_n + LPS_STENCIL_PAD]
     float *press = nullptr, *q = nullptr, *rho_pred = nullptr;
     uint32_t *mk_mask = nullptr, *mk_ok = nullptr;  // recorded candidate masks [LPS_MC_ROWS][cap_n], flags
     uint2* mk_cq = nullptr;                         // compact predictions [cap_n + LPS_STENCIL_PAD]
@@ -414,6 +415,7 @@ int ensure_particles(lps_ctx* c, uint32_t n) {
     if ((st = dalloc(&c->fp, 3 * n))) return st;
     if ((st = dalloc(&c->press, 3 * n))) return st;
     if ((st = dalloc(&c->q, n))) return st;
+    if ((st = dalloc(&c->xq, (size_t)n + LPS_STENCIL_PAD))) return st;
     if ((st = dalloc(&c->rho_pred, 3 * n))) return st;
     if ((st = dalloc(&c->mk_mask, (size_t)LPS_MC_ROWS * n))) return st;
     if ((st = dalloc(&c->mk_ok, n))) return st;
@@ -434,7 +436,7 @@ void free_all(lps_ctx* c) {
     }
     dfree(c->keysA); dfree(c->valsA); dfree(c->keysB); dfree(c->valsB);
     dfree(c->skeys); dfree(c->perm); dfree(c->hist); dfree(c->digit_total); dfree(c->bbox_part);
-    dfree(c->posrho); dfree(c->force); dfree(c->fp); dfree(c->press); dfree(c->q); dfree(c->rho_pred);
+    dfree(c->posrho); dfree(c->force); dfree(c->fp); dfree(c->press); dfree(c->q); dfree(c->xq); dfree(c->rho_pred);
     dfree(c->mk_mask); dfree(c->mk_ok); dfree(c->mk_cq);
     dfree(c->run_end); dfree(c->cell_lb); dfree(c->block_max);
     dfree(c->tables); dfree(c->boxes); dfree(c->spheres); dfree(c->planes); dfree(c->grid);
@@ -639,21 +641,21 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     if (mark) flag_slot(c, kk)[0] = FLAG_PENDING;
     {
         Timed t(c, "pci_density");
-        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->q, kk, feat,
+        lps_dev::launch_pci_density(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->press, c->xq, kk, feat,
                                     c->corr_threads);
     }
     if (kk == 1u && seeded) {
         // the first correction's pressure update is replaced by the seed
         Timed t(c, "mlp_seed");
-        lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->cap_n, c->n, c->press, c->q,
+        lps_dev::launch_mlp_seed(s, c->d_mlp_packed, c->mlp_tscale, c->feat, c->cap_n, c->n, c->press, c->xq,
                                  c->seed, c->rho_pred, c->pprev[c->cur], LPS_WARM_KAPPA,
                                  delta * c->params.rest_density, c->params.negative_pressure_factor == 0.f ? 1 : 0,
                                  c->grid, scenes(c), rule, c->params.rest_density);
     }
     {
         Timed t(c, "pci_pressure");
-        lps_dev::launch_pci_pressure(s, a, c->xpred[(kk - 1u) & 1u], c->rho_pred, c->q, c->fp, c->xpred[kk & 1u],
-                                     kk, rule, c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS), c->corr_threads);
+        lps_dev::launch_pci_pressure(s, a, c->xq, c->rho_pred, c->fp, c->xpred[kk & 1u], kk, rule,
+                                     c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS), c->corr_threads);
     }
 }
 

@@ -104,10 +104,11 @@ void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float*
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q);
 // rho_pred and press: three buffers of a.mk.stride slots each; correction k
 // writes the scene's buffer buf_of[k % 64] (and reads the previous pressure
-// from buf_of[(k - 1) % 64]); q is a single buffer
+// from buf_of[(k - 1) % 64]); xq receives (prediction xyz, p / rho*^2), the
+// pressure pass's single gather per candidate (a.n + LPS_STENCIL_PAD slots)
 // threads: workgroup size of the correction kernels (64, 128 or LPS_BLOCK)
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, float* feat, uint32_t threads);
+                        float4* xq, uint32_t k, float* feat, uint32_t threads);
 
 // ---- learned warm start (lps_mlp.hip)
 // Network parameters, fp32 master copy in this order: W1 [64][16], b1 [64],
@@ -131,13 +132,14 @@ struct RowMap {
 // y_i; the first correction's pressure (buffer buf_of[1] of press, stride
 // fs) becomes y_i, or with negative pressures disabled (clamp_nonneg)
 // max(p1_i, min(y_i, kappa * pprev_i)), p1_i the pressure that correction
-// computed; q_i = p_i / rho*_i^2.  No-op once the step has stopped or failed.
+// computed; the w slot of xq_i becomes p_i / rho*_i^2.  No-op once the step
+// has stopped or failed.
 // feat: feature-major, feature f of slot i at f * fs + i
 // skips the scenes whose loop stops at the first correction (rule, rest_density)
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
-                     float* press, float* q, float* seed, const float* rho_pred, const float* pprev, float kappa,
-                     float pscale, int clamp_nonneg, const DevGrid* g, const Scenes& sc, const LoopRule& rule,
-                     float rest_density);
+                     float* press, float4* xq, float* seed, const float* rho_pred, const float* pprev,
+                     float kappa, float pscale, int clamp_nonneg, const DevGrid* g, const Scenes& sc,
+                     const LoopRule& rule, float rest_density);
 // BF16 inference weights (k_mlp_seed's LDS layout) from the FP32 master copy
 size_t mlp_packed_bytes();
 void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed);
@@ -152,10 +154,10 @@ void launch_adam(hipStream_t s, float* params, float* grad, float* m1, float* m2
 void launch_feature_stats(hipStream_t s, const float* x, uint64_t n, uint32_t cols, double* acc);
 void launch_collect(hipStream_t s, const float* feat, uint32_t fs, const float* press, float pscale, uint32_t n,
                     uint32_t stride, uint32_t offset, uint32_t count, float* X, float* T, uint64_t base);
-// also applies the loop rule of correction k (block 0) and answers host_flag
-void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                         const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
-                         uint32_t* host_flag, uint32_t threads);
+// also applies the loop rule of correction k (block 0) and answers host_flag;
+// xq is the predicted-density pass's (prediction, p / rho*^2)
+void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xq, const float* rho_pred, float4* fp,
+                         float4* xpred_next, uint32_t k, const LoopRule& rule, uint32_t* host_flag, uint32_t threads);
 // PCISPH (fp given): fp and press hold three buffers of a.mk.stride slots;
 // each scene reads the one its loop applied (SceneGrid::final_buf).  SPH:
 // press is the density pass's pressure.  pprev receives the applied pressure.

@@ -1276,7 +1276,7 @@ template <bool FEAT, int WG>
 __global__ __launch_bounds__(WG) void k_pci_density(
     const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
     SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
-    float* __restrict__ q, uint32_t k, float* __restrict__ feat, const float4* __restrict__ posrho,
+    float4* __restrict__ xq, uint32_t k, float* __restrict__ feat, const float4* __restrict__ posrho,
     lps_dev::Scenes S, lps_dev::NbMasks mk, uint32_t nlogical) {
     const uint32_t slot = (k - 1u) % LPS_MAX_ITER_SLOTS;
     // a correction queued past the stop, or of a dead step, does nothing
@@ -1322,7 +1322,8 @@ __global__ __launch_bounds__(WG) void k_pci_density(
         const float p = pold + dp;
         rho_pred[ob + i] = rp;
         press[ob + i] = p;
-        q[i] = p / (rp * rp);
+        // the pressure pass gathers the prediction and p / rho*^2 in one load
+        xq[i] = make_float4(xi.x, xi.y, xi.z, p / (rp * rp));
         if (FEAT && k == 1u) {
             float* fr = feat + i;  // feature-major, as in k_forces
             const size_t fs = mk.stride;
@@ -1494,8 +1495,8 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
 // prediction (pciIntegrate + pciHandleCollisions) from f + fp.
 template <int WG>
 __global__ __launch_bounds__(WG) void k_pci_pressure(
-    const float4* __restrict__ pos, const float4* __restrict__ xpred, const float* __restrict__ rho_pred,
-    const float* __restrict__ q, DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
+    const float4* __restrict__ pos, const float4* __restrict__ xq, const float* __restrict__ rho_pred, DevGrid* g,
+    const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
@@ -1522,8 +1523,9 @@ __global__ __launch_bounds__(WG) void k_pci_pressure(
     const lps_sphere* spheres = spheres0 + B.info.sphere0;
     const lps_plane* planes = planes0 + B.info.plane0;
     const float4 xa = pos[i];
-    const float4 xi = xpred[i];
-    const float tf = q[i];
+    // the predicted-density pass's (prediction, p / rho*^2)
+    const float4 xi = xq[i];
+    const float tf = xi.w;
     // accepted pairs have d < 2^40; the predictions' coordinates were checked
     // when they were written (one instantiation of the stencil per case, so
     // the pair loop carries no mode test)
@@ -1531,10 +1533,11 @@ __global__ __launch_bounds__(WG) void k_pci_pressure(
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     auto pressure = [&](auto nocheck) {
         // the predicted-density pass of this correction recorded the masks
-        for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+        for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xq, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; },
-                             [&](uint32_t j) { return q[j]; },
-                             [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
+                             [](uint32_t) { return 0; },
+                             [&](uint32_t j, const float4& xqj, float s2, int, const float3& dl) {
+            const float qj = xqj.w;
             const float d = sqrt_rn(s2);
             float ux, uy, uz;
             pair_dir<decltype(nocheck)::value>(dl, d, i, j, fastdiv, ux, uy, uz);
@@ -1724,36 +1727,35 @@ void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
 
 template <int WG>
 static void pci_density_wg(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                           float* q, uint32_t k, float* feat) {
+                           float4* xq, uint32_t k, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     auto kern = (feat && k == 1u) ? k_pci_density<true, WG> : k_pci_density<false, WG>;
     kern<<<xcd_grid(nl * (LPS_BLOCK / WG)), WG, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press,
-                                                       q, k, feat, a.posrho, a.sc, a.mk, nl);
+                                                       xq, k, feat, a.posrho, a.sc, a.mk, nl);
 }
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
-                        float* q, uint32_t k, float* feat, uint32_t threads) {
-    if (threads == 64u) pci_density_wg<64>(s, a, xpred, rho_pred, press, q, k, feat);
-    else if (threads == 128u) pci_density_wg<128>(s, a, xpred, rho_pred, press, q, k, feat);
-    else pci_density_wg<LPS_BLOCK>(s, a, xpred, rho_pred, press, q, k, feat);
+                        float4* xq, uint32_t k, float* feat, uint32_t threads) {
+    if (threads == 64u) pci_density_wg<64>(s, a, xpred, rho_pred, press, xq, k, feat);
+    else if (threads == 128u) pci_density_wg<128>(s, a, xpred, rho_pred, press, xq, k, feat);
+    else pci_density_wg<LPS_BLOCK>(s, a, xpred, rho_pred, press, xq, k, feat);
 }
 
 template <int WG>
-static void pci_pressure_wg(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                            const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
-                            uint32_t* host_flag) {
+static void pci_pressure_wg(hipStream_t s, const PhysArgs& a, const float4* xq, const float* rho_pred, float4* fp,
+                            float4* xpred_next, uint32_t k, const LoopRule& rule, uint32_t* host_flag) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
     k_pci_pressure<WG><<<xcd_grid(nl * (LPS_BLOCK / WG)), WG, 0, s>>>(
-        a.pos, xpred, rho_pred, q, a.g, a.cell_lb, a.c, a.tables, fp, a.force, a.posrho, a.vel, a.vh, a.first,
-        a.boxes, a.spheres, a.planes, xpred_next, k, a.sc, a.mk, rule, host_flag, nl);
+        a.pos, xq, rho_pred, a.g, a.cell_lb, a.c, a.tables, fp, a.force, a.posrho, a.vel, a.vh, a.first, a.boxes,
+        a.spheres, a.planes, xpred_next, k, a.sc, a.mk, rule, host_flag, nl);
 }
 
-void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xpred, const float* rho_pred,
-                         const float* q, float4* fp, float4* xpred_next, uint32_t k, const LoopRule& rule,
-                         uint32_t* host_flag, uint32_t threads) {
-    if (threads == 64u) pci_pressure_wg<64>(s, a, xpred, rho_pred, q, fp, xpred_next, k, rule, host_flag);
-    else if (threads == 128u) pci_pressure_wg<128>(s, a, xpred, rho_pred, q, fp, xpred_next, k, rule, host_flag);
-    else pci_pressure_wg<LPS_BLOCK>(s, a, xpred, rho_pred, q, fp, xpred_next, k, rule, host_flag);
+void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xq, const float* rho_pred, float4* fp,
+                         float4* xpred_next, uint32_t k, const LoopRule& rule, uint32_t* host_flag,
+                         uint32_t threads) {
+    if (threads == 64u) pci_pressure_wg<64>(s, a, xq, rho_pred, fp, xpred_next, k, rule, host_flag);
+    else if (threads == 128u) pci_pressure_wg<128>(s, a, xq, rho_pred, fp, xpred_next, k, rule, host_flag);
+    else pci_pressure_wg<LPS_BLOCK>(s, a, xq, rho_pred, fp, xpred_next, k, rule, host_flag);
 }
 
 void launch_integrate(hipStream_t s, const PhysArgs& a, const float4* fp, float* pprev, const float* press,

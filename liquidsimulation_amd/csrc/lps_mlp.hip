@@ -269,7 +269,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_mlp_forward(const MlpPacked* __re
 // would spill).
 __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_mlp_seed(const MlpPacked* __restrict__ packed, float tscale,
                                                         const float* __restrict__ feat, uint32_t fs, uint32_t n,
-                                                        float* __restrict__ press, float* __restrict__ q,
+                                                        float* __restrict__ press, float4* __restrict__ xq,
                                                         float* __restrict__ seed, const float* __restrict__ rho_pred,
                                                         const float* __restrict__ pprev, float kappa,
                                                         float pscale, int clamp_nonneg, const DevGrid* g,
@@ -326,7 +326,8 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) 
                 }
                 const float rp = rho_pred[ob + p];
                 press[ob + p] = v;
-                q[p] = v / (rp * rp);
+                // the pressure pass reads p / rho*^2 from the w slot of the prediction copy
+                reinterpret_cast<float*>(xq)[4u * p + 3u] = v / (rp * rp);
             }
         }
     }
@@ -538,12 +539,12 @@ void launch_mlp_pack(hipStream_t s, const MlpArgs& m, void* packed) {
 }
 
 void launch_mlp_seed(hipStream_t s, const void* packed, float tscale, const float* feat, uint32_t fs, uint32_t n,
-                     float* press, float* q, float* seed, const float* rho_pred, const float* pprev, float kappa,
-                     float pscale, int clamp_nonneg, const DevGrid* g, const Scenes& sc, const LoopRule& rule,
-                     float rest_density) {
+                     float* press, float4* xq, float* seed, const float* rho_pred, const float* pprev,
+                     float kappa, float pscale, int clamp_nonneg, const DevGrid* g, const Scenes& sc,
+                     const LoopRule& rule, float rest_density) {
     // one block per particle block (n is a whole number of blocks)
     k_mlp_seed<<<(n + LPS_BLOCK - 1u) / LPS_BLOCK, LPS_BLOCK, 0, s>>>(static_cast<const MlpPacked*>(packed), tscale,
-                                                                      feat, fs, n, press, q, seed, rho_pred, pprev,
+                                                                      feat, fs, n, press, xq, seed, rho_pred, pprev,
                                                                       kappa, pscale, clamp_nonneg, g, sc, rule,
                                                                       rest_density);
 }
