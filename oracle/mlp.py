@@ -27,9 +27,11 @@ Here every sum is taken in float64 over the rounded operands; the kernels sum
 in FP32 in the matrix cores' order, so GPU parity is within a tolerance.
 
 Pins (tests/test_mlp_oracle.py): bf16() against torch's bfloat16 conversion
-(round to nearest even); the unrounded float64 gradient against central finite
-differences; the rounded forward and gradient equal the unrounded ones when
-every operand is exactly representable in bf16.
+(round to nearest even); the unrounded float64 forward pass against torch's
+float64 Linear/ReLU modules on the same packed parameters, and its gradient
+against torch autograd and central finite differences; the rounded forward and
+gradient equal the unrounded ones when every operand is exactly representable
+in bf16.
 """
 from __future__ import annotations
 

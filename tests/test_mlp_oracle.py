@@ -1,7 +1,8 @@
 """Pins of the warm-start network's CPU restatement (oracle/mlp.py): the BF16
-rounding against torch's conversion, the float64 gradient against finite
-differences, and the rounded paths against the exact ones where every operand
-is representable in BF16."""
+rounding against torch's conversion, the exact float64 forward pass and
+gradient against torch's float64 modules and autograd, the gradient against
+finite differences, and the rounded paths against the exact ones where every
+operand is representable in BF16."""
 import numpy as np
 import pytest
 
@@ -116,3 +117,52 @@ def test_rounding_moves_results_by_bf16_amounts():
     b = M.forward(p, fs, x, exact=True)["y"]
     rel = np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))
     assert 1e-4 < rel < 2e-2
+
+
+def _torch_net(torch, p, fs):
+    """The network restated with torch modules in float64: Linear(16, 64), ReLU,
+    Linear(64, 64), ReLU, Linear(64, 1) after (x - mean) * inv_std, with the
+    parameters taken from the packed vector in the library's layout (W1 [64][16],
+    b1, W2 [64][64], b2, W3 [64], b3)."""
+    W1, b1, W2, b2, w3, b3 = (np.asarray(a, np.float64) for a in M.unpack_f64(p))
+    net = torch.nn.Sequential(torch.nn.Linear(M.IN, M.HID), torch.nn.ReLU(), torch.nn.Linear(M.HID, M.HID),
+                              torch.nn.ReLU(), torch.nn.Linear(M.HID, 1)).double()
+    with torch.no_grad():
+        net[0].weight.copy_(torch.from_numpy(W1)); net[0].bias.copy_(torch.from_numpy(b1))
+        net[2].weight.copy_(torch.from_numpy(W2)); net[2].bias.copy_(torch.from_numpy(b2))
+        net[4].weight.copy_(torch.from_numpy(w3.reshape(1, M.HID)))
+        net[4].bias.copy_(torch.from_numpy(np.array([b3])))
+    mean = torch.from_numpy(np.asarray(fs[:M.IN], np.float64))
+    inv = torch.from_numpy(np.asarray(fs[M.IN:], np.float64))
+    return net, mean, inv
+
+
+def test_exact_forward_matches_torch_float64():
+    """The oracle's exact forward pass against torch's float64 modules on the
+    same packed parameters (pins the layout and the normalisation)."""
+    torch = pytest.importorskip("torch")
+    p, fs, ts, x, t = _problem(7, n=200)
+    net, mean, inv = _torch_net(torch, p, fs)
+    ref = net((torch.from_numpy(x) - mean) * inv).detach().numpy().reshape(-1)
+    ours = M.forward(p, fs, x, exact=True)["y"]
+    np.testing.assert_allclose(ours, ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("ow", [1.0, 3.0])
+def test_exact_gradient_matches_torch_autograd(ow):
+    """The oracle's exact gradient of mean(w (y - t / tscale)^2), w = ow where
+    the prediction is above the target, against torch autograd in float64."""
+    torch = pytest.importorskip("torch")
+    p, fs, ts, x, t = _problem(8, n=150)
+    net, mean, inv = _torch_net(torch, p, fs)
+    y = net((torch.from_numpy(x) - mean) * inv).reshape(-1)
+    d = y - torch.from_numpy(np.asarray(t, np.float64)) / ts
+    w = torch.where(d > 0, torch.tensor(ow, dtype=torch.float64), torch.tensor(1.0, dtype=torch.float64))
+    loss = torch.mean(w.detach() * d * d)
+    loss.backward()
+    ref = np.concatenate([net[0].weight.grad.numpy().ravel(), net[0].bias.grad.numpy(),
+                          net[2].weight.grad.numpy().ravel(), net[2].bias.grad.numpy(),
+                          net[4].weight.grad.numpy().ravel(), net[4].bias.grad.numpy()])
+    g, l0 = M.grad(p, fs, ts, x, t, exact=True, over_weight=ow)
+    assert abs(l0 - float(loss.detach())) <= 1e-12 * max(1.0, abs(l0))
+    np.testing.assert_allclose(g, ref, rtol=1e-10, atol=1e-12)
