@@ -156,14 +156,16 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(
             "windows_ms": windows, "bounds": bounds, "warmup": warmup}
 
 
-def run_ensemble(ens, steps: int, warmup: int, weights=None, profile: bool = False):
+def run_ensemble(ens, steps: int, warmup: int, weights=None, profile: bool = False, always: bool = False):
     """All scenes of an ensemble in one context: warm-up, then `steps` timed
-    steps (HIP events on the library stream)."""
+    steps (HIP events on the library stream).  With weights the learned seed
+    runs with the library's fallback, or on every step (always)."""
     import torch
     from liquidsimulation_amd.solver import Ensemble
-    from liquidsimulation_amd.abi import WARM_LEARNED, WARM_NONE
+    from liquidsimulation_amd.abi import WARM_LEARNED, WARM_LEARNED_ALWAYS, WARM_NONE
     from liquidsimulation_amd import warm_start
-    e = Ensemble(ens.params.replace(warm_start=WARM_LEARNED if weights is not None else WARM_NONE))
+    mode = WARM_NONE if weights is None else (WARM_LEARNED_ALWAYS if always else WARM_LEARNED)
+    e = Ensemble(ens.params.replace(warm_start=mode))
     if weights is not None:
         warm_start.apply(e, weights)
     e.set_scenes([m.x for m in ens.members], [m.v for m in ens.members], gravity=ens.gravity)
@@ -237,9 +239,11 @@ def bench_ensemble(args):
     w = warm_start.load() if (args.mode == "both" and os.path.exists(warm_start.WEIGHTS)) else None
     plain = run_ensemble(ens, steps, args.warmup, None, not args.no_profile)
     learned = run_ensemble(ens, steps, args.warmup, w, False) if w is not None else None
+    learned_always = run_ensemble(ens, steps, args.warmup, w, False, always=True) if w is not None else None
     alone = run_members_alone(ens, max(20, steps // 10), args.warmup)
-    value_mode, value_reason = choose_value(plain, {"learned": learned})
-    head = learned if value_mode == "learned" else plain
+    modes = {"learned": learned, "learned_always": learned_always}
+    value_mode, value_reason = choose_value(plain, modes)
+    head = plain if value_mode == "plain" else modes[value_mode]
     out = {
         "metric": "particle-steps/s of an ensemble of independent PCISPH dam breaks in one context, 1 GPU",
         "value": head["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
@@ -250,7 +254,7 @@ def bench_ensemble(args):
                    "particles_max": int(max(m.n for m in ens.members)), "dt": ens.dt,
                    "warm_start": "learned" if learned is not None else "none"},
         "value_mode": value_mode, "value_reason": value_reason,
-        "plain": plain, "learned": learned,
+        "plain": plain, "learned": learned, "learned_always": learned_always,
         "scenes_one_at_a_time": alone,
         "speedup_batched_vs_one_at_a_time": plain["value"] / alone["value"],
     }
