@@ -8,7 +8,7 @@
  *   LiquidPhysics/cl_kernels/SPHKernels.cl through ParallelComputationInterface
  *   (LiquidPhysics/include/Parallelization/ParallelComputationInterface.h:35-58).
  * A solver adapter (a fourth ParallelizationType) calls these entry points in
- * place of the OpenCL launch sequence; see INTEGRATION.md for the binding.
+ * place of the OpenCL launch sequence (include/lps_liphen_adapter.hpp is one).
  *
  * Conventions: plain C types, host arrays owned by the caller and copied in
  * or out; device memory owned by the context.  Every function returns an
