@@ -493,7 +493,7 @@ void enqueue_grid(lps_ctx* c) {
     }
     {
         Timed t(c, "keys");
-        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, scenes(c), c->keysA, c->valsA, c->block_max);
+        lps_dev::launch_keys(s, c->pos[c->cur], n, c->grid, scenes(c), c->keysA, c->valsA, c->block_max, c->hist);
     }
     for (uint32_t p = 0; p < c->passes; p++) {
         Timed t(c, "radix_pass");

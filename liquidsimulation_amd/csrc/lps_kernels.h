@@ -89,9 +89,9 @@ void launch_scene_energy(hipStream_t s, const Scenes& sc, const uint32_t* part, 
 // to energy[2 * scene .. + 1] when energy is given
 void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32_t* part, float h, uint32_t cell_cap,
                        double* energy);
+// keys, and the first radix pass's histogram into hist (that pass skips its own)
 void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
-                 uint32_t* vals,
-                 uint32_t* tile_max);
+                 uint32_t* vals, uint32_t* tile_max, uint32_t* hist);
 uint32_t radix_tiles(uint32_t n);
 void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist, uint32_t* digit_total);

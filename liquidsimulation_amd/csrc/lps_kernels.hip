@@ -811,29 +811,57 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_grid_one(DevGrid* g, lps_dev::Sce
 // calcGridIndices (SPHKernels.cl:66-82) within the scene's grid, offset by the
 // scene's first cell; empty slots get the scene's empty-slot cell so they sort
 // after its particles.
+// Cell keys (calcGridIndices), one radix tile of LPS_RADIX_TILE slots per
+// block in the radix kernels' item order, and the tile's histogram of the
+// first digit (the first radix pass then starts at its scan).
 __global__ __launch_bounds__(LPS_BLOCK) void k_keys(const float4* __restrict__ pos, uint32_t n, const DevGrid* g,
                                                     lps_dev::Scenes S, uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ tile_max) {
+                                                    uint32_t* __restrict__ vals, uint32_t* __restrict__ tile_max,
+                                                    uint32_t* __restrict__ hist, uint32_t ntiles) {
     if (step_dead(g)) return;
-    const uint32_t i = blockIdx.x * LPS_BLOCK + threadIdx.x;
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0u;
     // the cell table's per-tile maxima of the run ends start at zero; the
     // reorder writes those of the non-empty tiles
-    const uint32_t ntiles = (g->total_cells + LPS_SCAN_TILE) / LPS_SCAN_TILE;
-    for (uint32_t b = i; b < ntiles; b += gridDim.x * LPS_BLOCK) tile_max[b] = 0u;
-    if (i >= n) return;
-    const BlockScene B = block_scene(S, blockIdx.x);
-    const SceneGrid* sg = S.grid + B.s;
-    uint32_t key = sg->cell_base + sg->cells;
-    if (i < B.info.slot0 + B.info.count) {
-        const GridView G = grid_view(g, sg);
-        const float4 p = pos[i];
-        const uint32_t xg = (uint32_t)truncf((p.x + G.off0) / G.gs);
-        const uint32_t yg = (uint32_t)truncf((p.y + G.off1) / G.gs);
-        const uint32_t zg = (uint32_t)truncf((p.z + G.off2) / G.gs);
-        key = G.base + xg + (uint32_t)G.gx * yg + G.gxy * zg;
+    const uint32_t ncell_tiles = (g->total_cells + LPS_SCAN_TILE) / LPS_SCAN_TILE;
+    for (uint32_t b = blockIdx.x * LPS_BLOCK + threadIdx.x; b < ncell_tiles; b += gridDim.x * LPS_BLOCK)
+        tile_max[b] = 0u;
+    __syncthreads();
+    const uint32_t tile = blockIdx.x;
+    // the tile's four positions are loaded before any key is formed
+    float4 p[LPS_RADIX_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const uint32_t i = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
+        p[r] = i < n ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    keys[i] = key;
-    vals[i] = i;
+#pragma unroll
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const uint32_t i = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t key = 0u;
+        if (valid) {
+            // slot i lies in particle block i / LPS_BLOCK, uniform over the round
+            const BlockScene B = block_scene(S, i / LPS_BLOCK);
+            const SceneGrid* sg = S.grid + B.s;
+            key = sg->cell_base + sg->cells;
+            if (i < B.info.slot0 + B.info.count) {
+                const GridView G = grid_view(g, sg);
+                const uint32_t xg = (uint32_t)truncf((p[r].x + G.off0) / G.gs);
+                const uint32_t yg = (uint32_t)truncf((p[r].y + G.off1) / G.gs);
+                const uint32_t zg = (uint32_t)truncf((p[r].z + G.off2) / G.gs);
+                key = G.base + xg + (uint32_t)G.gx * yg + G.gxy * zg;
+            }
+            keys[i] = key;
+            vals[i] = i;
+        }
+        const uint32_t d = key & 0xFFu;
+        const uint64_t act = __ballot(valid);
+        const uint64_t peers = match_digit(d, act);
+        if (valid && lane_id() == (uint32_t)__ffsll((long long)peers) - 1u) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
 }
 
 // ---- stable LSD radix sort by key, 8-bit digits, reduce-then-scan ----------
@@ -1675,17 +1703,19 @@ void launch_grid_setup(hipStream_t s, DevGrid* g, const Scenes& sc, const uint32
     }
 }
 
-void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
-                 uint32_t* vals, uint32_t* tile_max) {
-    k_keys<<<cdiv(n, LPS_BLOCK), LPS_BLOCK, 0, s>>>(pos, n, g, sc, keys, vals, tile_max);
-}
-
 uint32_t radix_tiles(uint32_t n) { return cdiv(n, LPS_RADIX_TILE); }
+
+void launch_keys(hipStream_t s, const float4* pos, uint32_t n, const DevGrid* g, const Scenes& sc, uint32_t* keys,
+                 uint32_t* vals, uint32_t* tile_max, uint32_t* hist) {
+    const uint32_t nt = radix_tiles(n);
+    k_keys<<<nt, LPS_BLOCK, 0, s>>>(pos, n, g, sc, keys, vals, tile_max, hist, nt);
+}
 
 void launch_radix_pass(hipStream_t s, const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout,
                        uint32_t n, uint32_t shift, DevGrid* g, uint32_t* hist, uint32_t* digit_total) {
     const uint32_t nt = radix_tiles(n);
-    k_radix_hist<<<nt, LPS_BLOCK, 0, s>>>(kin, n, shift, g, hist, nt);
+    // the first digit's histogram comes from the key kernel
+    if (shift != 0u) k_radix_hist<<<nt, LPS_BLOCK, 0, s>>>(kin, n, shift, g, hist, nt);
     k_radix_scan<<<256, LPS_BLOCK, 0, s>>>(hist, nt, digit_total, g, shift);
     k_radix_scatter<<<nt, LPS_BLOCK, 0, s>>>(kin, vin, kout, vout, n, shift, g, hist, nt, digit_total);
 }
