@@ -86,9 +86,9 @@ struct WarmFallback {
     double win[WS_WINDOW] = {};
     uint32_t n = 0, head = 0;
     int32_t prev_its = 0;
-    int8_t prev_seeded = -1;     // mode of the last recorded step, -1 none
+    int8_t prev_seeded = -1;     // mode of the last recorded step; -1 after new weights or a new state
     bool on = true;
-    uint64_t steps = 0;          // steps recorded since the weights or the state changed
+    uint64_t steps = 0;          // steps recorded since the weights changed
     double ema_de_plain = 0.0;   // running relative energy change of zero-start steps
 
     void push(double d) {
@@ -1110,6 +1110,7 @@ int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const f
     lps_dev::launch_bbox(c->stream, c->pos[0], c->vel[0], c->der.mass, n, scenes(c), c->bbox_part);
     HIPC(hipStreamSynchronize(c->stream));
     c->e_prev_valid = false;
+    c->wf.prev_seeded = -1;  // a new state: its first step pairs with no earlier one
     return LPS_OK;
 }
 
