@@ -249,3 +249,84 @@ def test_corrections_past_64_use_their_own_error(gpu, oracle):
         assert ulp_diff(q["x"], a.x) == 0
         assert ulp_diff(q["v"], a.v) == 0
         s.close()
+
+
+# Parameter settings the configured scenes do not reach, each run for 12 steps
+# on both sides (the trajectories, iteration counts and the converged flags
+# stay bit-identical): a negative-pressure factor between 0 and 1 (both
+# branches of the pressure update), viscosity and surface tension off, an
+# inelastic frictionless wall (restitution 0, friction 0) with an obstacle box
+# inside the boundary box, a loop capped at 2 corrections without the stall
+# rule, and a tolerance ten times stricter.
+VARIANTS = {
+    "npf_half": dict(negative_pressure_factor=0.5),
+    "no_viscosity_no_tension": dict(viscosity=0.0, surface_tension=0.0),
+    "inelastic_frictionless_with_box_obstacle": dict(restitution=0.0, friction=0.0),
+    "loop_capped_at_2": dict(min_iterations=1, max_iterations=2, stall_iterations=0),
+    "strict_tolerance": dict(max_density_error_ratio=0.001),
+}
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+@pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
+def test_parameter_variants_multi_step(gpu, oracle, method, variant):
+    from liquidsimulation_amd.abi import Box
+    sc = scenes.small_block(9, 11, 7, method=method, dt=0.002, **VARIANTS[variant])
+    obstacle = variant.startswith("inelastic")
+    if obstacle:
+        # an obstacle box just past the block's leading face, in its path
+        xmax = float(sc.x[:, 0].max())
+        zc = float(sc.x[:, 2].mean())
+        sc.colliders.boxes = [sc.colliders.boxes[0],
+                              Box(center=(xmax + 0.036, 0.06, zc), half=(0.03, 0.06, 0.05), boundary=False)]
+    st = oracle.State.from_scene(sc.x)
+    st.v[:, 0] = np.float32(0.4)  # moving towards the walls and the obstacle
+    st.vh[:, 0] = np.float32(0.4)
+    st.first[:] = 0
+    a = st.copy()
+    its, conv = [], []
+    for _ in range(12):
+        tr = oracle.step_ocl(sc.params, sc.colliders, sc.dt, a)
+        assert tr.status == 0
+        its.append(tr.iterations)
+        conv.append(tr.converged)
+    if obstacle:
+        # the obstacle changes the trajectory (the same steps without it differ)
+        import dataclasses
+        free = st.copy()
+        col = dataclasses.replace(sc.colliders, boxes=sc.colliders.boxes[:1])
+        for _ in range(12):
+            oracle.step_ocl(sc.params, col, sc.dt, free)
+        assert not np.array_equal(free.x, a.x)
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 12)
+    p = s.get_particles()
+    print(f"[{variant}] iterations gpu", [x.iterations for x in stats], "oracle", its, flush=True)
+    assert [x.iterations for x in stats] == its
+    if method == METHOD_PCISPH:
+        assert [x.converged for x in stats] == conv
+    for k, ref in (("x", a.x), ("v", a.v), ("vhalf", a.vh)):
+        assert ulp_diff(p[k], ref) == 0, k
+    np.testing.assert_array_equal(p["first"], a.first)
+    s.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 105, 255, 256, 257, 513])
+@pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
+def test_particle_counts_around_the_block_size(gpu, oracle, method, n):
+    """Counts below, at and past the 256-slot particle block (and a single
+    particle): 5 steps bit-identical, state and iteration counts."""
+    sc = scenes.small_block(9, 9, 9, method=method, dt=0.002)
+    sc.x = sc.x[:n].copy()
+    st = oracle.State.from_scene(sc.x)
+    a = st.copy()
+    its = []
+    for _ in range(5):
+        tr = oracle.step_ocl(sc.params, sc.colliders, sc.dt, a)
+        assert tr.status == 0
+        its.append(tr.iterations)
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 5)
+    p = s.get_particles()
+    assert [x.iterations for x in stats] == its
+    for k, ref in (("x", a.x), ("v", a.v), ("vhalf", a.vh)):
+        assert ulp_diff(p[k], ref) == 0, k
+    s.close()
