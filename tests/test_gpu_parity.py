@@ -330,3 +330,46 @@ def test_particle_counts_around_the_block_size(gpu, oracle, method, n):
     for k, ref in (("x", a.x), ("v", a.v), ("vhalf", a.vh)):
         assert ulp_diff(p[k], ref) == 0, k
     s.close()
+
+
+@pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
+def test_grid_growth_past_the_initial_cell_table(gpu, oracle, method):
+    """No boundary box, and six particles leaving the block at 40 m/s along
+    +-x, +-y, +-z: the bounding box outgrows the cell table sized at upload
+    (about eight times the block's cells), the step that finds it too small
+    is issued again after the table grows, and 10 steps stay bit-identical."""
+    sc = scenes.small_block(9, 9, 9, method=method, dt=0.002)
+    sc.colliders.boxes = []
+    x = sc.x.copy()
+    lo, hi, mid = x.min(0), x.max(0), x.mean(0)
+    out = np.repeat(mid[None, :], 6, 0).astype(np.float32)
+    vel = np.zeros((6, 3), np.float32)
+    h = sc.params.particle_radius * sc.params.kernel_radius_factor
+    for k in range(3):
+        out[2 * k, k] = lo[k] - 2 * h
+        out[2 * k + 1, k] = hi[k] + 2 * h
+        vel[2 * k, k] = -40.0
+        vel[2 * k + 1, k] = 40.0
+    sc.x = np.concatenate([x, out]).astype(np.float32)
+    st = oracle.State.from_scene(sc.x)
+    st.v[-6:] = vel
+    st.vh[-6:] = vel
+    st.first[-6:] = 0
+    cells0 = 1
+    for k in range(3):
+        cells0 *= int((float(sc.x[:, k].max()) - float(sc.x[:, k].min())) / h) + 2
+    a = st.copy()
+    its = []
+    for _ in range(10):
+        tr = oracle.step_ocl(sc.params, sc.colliders, sc.dt, a)
+        assert tr.status == 0
+        its.append(tr.iterations)
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 10)
+    g = s.get_grid(cell_cap=1)
+    p = s.get_particles()
+    print(f"cells at upload {cells0}, after 10 steps {g['cell_count']} (oracle {tr.cell_count})", flush=True)
+    assert g["cell_count"] == tr.cell_count and tr.cell_count > 9 * cells0 + 64
+    assert [x.iterations for x in stats] == its
+    for k, ref in (("x", a.x), ("v", a.v), ("vhalf", a.vh)):
+        assert ulp_diff(p[k], ref) == 0, k
+    s.close()
