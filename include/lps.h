@@ -53,7 +53,8 @@ typedef enum lps_warm_start {
 /* Fluid and solver parameters.  Field meanings follow the reference setters
  * (SPHSolver.h:57-69, PCISPHSolver.h:19-20); derived quantities (kernel radius,
  * lookup tables, particle mass, PCISPH delta) follow SPHSolver.cpp:480-523 and
- * PCISPHSolver.cpp:286-328. */
+ * PCISPHSolver.cpp:286-328.  Every float field must be finite, and the radius,
+ * kernel radius factor and rest density positive (else LPS_ERR_INVALID). */
 typedef struct lps_params {
     uint32_t abi_version;            /* LPS_ABI_VERSION */
     int32_t method;                  /* lps_method */
@@ -201,7 +202,8 @@ int lps_get_tuning(lps_ctx* ctx, lps_tuning* tuning);
 int lps_autotune(lps_ctx* ctx, float dt, uint32_t steps, lps_tuning* chosen);
 
 /* One or more steps of PhysicSolver::update (PhysicSolver.cpp:12-21) on the device.
- * stats may be NULL or hold nsteps entries.  Returns the first failing status. */
+ * stats may be NULL or hold nsteps entries.  dt must be positive and finite
+ * (else LPS_ERR_INVALID).  Returns the first failing status. */
 int lps_step(lps_ctx* ctx, float dt, uint32_t nsteps, lps_step_stats* stats);
 
 /* Parity hooks: the grid and fields of the last step (sorted order).

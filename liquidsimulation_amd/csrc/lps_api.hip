@@ -332,6 +332,14 @@ int check_params(const lps_params* p) {
         return LPS_ERR_INVALID;
     if (p->method == LPS_METHOD_PCISPH && (p->max_iterations < 1 || p->max_iterations > 10000)) return LPS_ERR_INVALID;
     if (p->warm_start < LPS_WARM_NONE || p->warm_start > LPS_WARM_LEARNED_ALWAYS) return LPS_ERR_INVALID;
+    // every other float parameter must be finite (a non-finite one would
+    // only show later, as a failed step)
+    const float fs[] = {p->gravity[0], p->gravity[1], p->gravity[2], p->particle_radius, p->kernel_radius_factor,
+                        p->rest_density, p->stiffness, p->negative_pressure_factor, p->viscosity,
+                        p->surface_tension, p->surface_tension_threshold, p->restitution, p->friction,
+                        p->mass_override, p->max_density_error_ratio};
+    for (float f : fs)
+        if (!std::isfinite(f)) return LPS_ERR_INVALID;
     return LPS_OK;
 }
 
@@ -1304,7 +1312,7 @@ int lps_step(lps_ctx* c, float dt, uint32_t nsteps, lps_step_stats* stats) {
     if (!c) return LPS_ERR_INVALID;
     if (c->n_real == 0) return LPS_ERR_STATE;
     if (c->n == 0) return LPS_ERR_STATE;
-    if (!(dt > 0.f)) return LPS_ERR_INVALID;
+    if (!(dt > 0.f) || !std::isfinite(dt)) return LPS_ERR_INVALID;
     const bool pci = c->params.method == LPS_METHOD_PCISPH;
     // A failed step leaves the state as it was (its kernels no-op and the
     // reorder copies the identity), so after a grid overflow the cell table is
@@ -1433,7 +1441,7 @@ int lps_get_tuning(lps_ctx* c, lps_tuning* t) {
 }
 
 int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
-    if (!c || !(dt > 0.f) || steps == 0 || steps > 1000) return LPS_ERR_INVALID;
+    if (!c || !(dt > 0.f) || !std::isfinite(dt) || steps == 0 || steps > 1000) return LPS_ERR_INVALID;
     if (c->n_real == 0) return LPS_ERR_STATE;
     if (c->prof) return LPS_ERR_STATE;
     const hipStream_t s = c->stream;

@@ -63,14 +63,13 @@ class SPHSolver:
         return self._params
 
     def _apply(self, **kw):
-        self._params = self._params.replace(**kw)
-        cp = self._params.to_c()
-        check(lib().lps_set_params(self._ctx, C.byref(cp)), "lps_set_params")
+        self.set_params(self._params.replace(**kw))
 
     def set_params(self, params: Params):
-        self._params = params
+        # kept only once the library accepted them
         cp = params.to_c()
         check(lib().lps_set_params(self._ctx, C.byref(cp)), "lps_set_params")
+        self._params = params
 
     def setGravity(self, g):
         self._apply(gravity=tuple(float(v) for v in g))

@@ -1,0 +1,81 @@
+"""Error reporting of the C ABI on a device, for conditions the host or the
+library's own checks turn into a status (the reference has no error channel:
+its solver has no failure path, LiquidPhysics/src/PhysicSolver.cpp:12-21).
+
+* A non-finite coordinate in the uploaded state: the bounding-box partials
+  flag it, the step's grid set-up marks the step failed, every later kernel
+  of the step returns at once, and lps_step returns LPS_ERR_NONFINITE with the
+  state as it was.  A clean upload afterwards steps normally.
+* A step before any particles were uploaded: LPS_ERR_STATE.
+* A time step that is not positive and finite, and parameters out of range
+  or not finite: LPS_ERR_INVALID, nothing changed.  (Fewer than one minimum
+  correction and a negative stall count are not errors: the loop takes them
+  as 1 and as no stall rule.)
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from liquidsimulation_amd import scenes
+from liquidsimulation_amd._lib import LPSError, lib
+from liquidsimulation_amd.abi import METHOD_PCISPH, METHOD_SPH
+from liquidsimulation_amd.solver import solver_for
+
+pytestmark = pytest.mark.gpu
+
+LPS_ERR_INVALID, LPS_ERR_NONFINITE, LPS_ERR_STATE = 1, 4, 7
+
+
+@pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
+def test_nonfinite_upload_fails_the_step_and_keeps_the_state(gpu, method):
+    sc = scenes.small_block(8, 8, 8, method=method)
+    s = solver_for(sc.params)
+    s.set_colliders(sc.colliders)
+    x = sc.x.copy()
+    x[17, 1] = np.float32(np.nan)
+    s.set_particles(x)
+    with pytest.raises(LPSError) as err:
+        s.update(sc.dt, 3)
+    assert err.value.status == LPS_ERR_NONFINITE
+    p = s.get_particles()
+    np.testing.assert_array_equal(np.isnan(p["x"]), np.isnan(x))
+    np.testing.assert_array_equal(p["x"][~np.isnan(x)], x[~np.isnan(x)])
+    # a clean state steps again in the same context
+    s.set_particles(sc.x)
+    st = s.update(sc.dt, 2)
+    assert all(t.status == 0 for t in st) and np.isfinite(s.get_particles()["x"]).all()
+    s.close()
+
+
+def test_step_before_particles_and_bad_time_steps(gpu):
+    sc = scenes.small_block(6, 6, 6)
+    s = solver_for(sc.params)
+    stats = (C.c_byte * 256)()
+    assert lib().lps_step(s._ctx, C.c_float(sc.dt), 1, C.cast(stats, C.c_void_p)) == LPS_ERR_STATE
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x)
+    for dt in (0.0, -1e-3, float("nan"), float("inf")):
+        with pytest.raises(LPSError) as err:
+            s.update(dt, 1)
+        assert err.value.status == LPS_ERR_INVALID, dt
+    # nothing moved
+    np.testing.assert_array_equal(s.get_particles()["x"], sc.x)
+    s.close()
+
+
+@pytest.mark.parametrize("bad", [dict(max_iterations=0), dict(rest_density=-1.0), dict(kernel_radius_factor=0.0),
+                                 dict(particle_radius=float("nan")), dict(viscosity=float("nan")),
+                                 dict(gravity=(0.0, float("-inf"), 0.0)), dict(method=7), dict(warm_start=9)])
+def test_invalid_parameters_are_refused_and_change_nothing(gpu, bad):
+    sc = scenes.small_block(6, 6, 6)
+    s = solver_for(sc.params)
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x)
+    with pytest.raises(LPSError) as err:
+        s.set_params(sc.params.replace(**bad))
+    assert err.value.status == LPS_ERR_INVALID
+    assert s.params == sc.params
+    st = s.update(sc.dt, 1)
+    assert st[0].status == 0
+    s.close()
