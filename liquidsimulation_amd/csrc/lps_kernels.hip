@@ -294,6 +294,57 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
     }
 }
 
+// One stencil row [b, e) of for_neighbors_masked: the recorded first window
+// (replay), then pass 1 and pass 2 over the remaining windows of 32.
+template <int VB, class Acc, class Pre, class F>
+__device__ __forceinline__ void visit_row(const char* __restrict__ xb, const CompactFilter& CF, uint32_t b, uint32_t e,
+                                          uint32_t* __restrict__ row, uint32_t i, bool replay, bool record, float px,
+                                          float py, float pz, float s_max, float s_relax, Acc& accept_extra, Pre& pre,
+                                          F& fn) {
+    const char* __restrict__ cb = reinterpret_cast<const char*>(CF.cq);
+    auto keep = [&](const float4& q) {
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) <= s_relax;
+    };
+    uint32_t w0 = b;
+    if (replay) {
+        visit_window<VB>(xb, b, row[i], px, py, pz, s_max, accept_extra, pre, fn);
+        w0 += 32u;
+    }
+    uint32_t first = 0u;
+    for (; w0 < e; w0 += 32u) {
+        const uint32_t n = min(32u, e - w0);
+        uint32_t mask = 0u;
+        if (cb) {
+            for (uint32_t u = 0; u < n; u += 8u) {
+                const char* p8 = cb + ((w0 + u) << 3);
+                uint2 e[8];
+#pragma unroll
+                for (int t = 0; t < 8; t++) e[t] = *reinterpret_cast<const uint2*>(p8 + 8 * t);
+                uint32_t m8 = 0u;
+#pragma unroll
+                for (int t = 7; t >= 0; t--) m8 = __builtin_amdgcn_alignbit(m8, compact_diff(e[t], CF.qi, CF.bias), 31u);
+                mask |= m8 << u;
+            }
+        } else {
+            for (uint32_t u = 0; u < n; u += 4u) {
+                const char* p8 = xb + ((w0 + u) << 4);
+                float4 q[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
+                uint32_t m8 = 0u;
+#pragma unroll
+                for (int t = 0; t < 4; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
+                mask |= m8 << u;
+            }
+        }
+        mask &= n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u;
+        if (w0 == b) first = mask;
+        visit_window<VB>(xb, w0, mask, px, py, pz, s_max, accept_extra, pre, fn);
+    }
+    if (record) row[i] = first;
+}
+
 template <int VB, class Acc, class Pre, class F>
 __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const uint32_t* __restrict__ lb, float ax,
                                                      float ay, float az, const float4* __restrict__ xs, float px,
@@ -301,68 +352,110 @@ __device__ __forceinline__ void for_neighbors_masked(const GridView& G, const ui
                                                      const lps_dev::NbMasks& M, uint32_t i, int mode,
                                                      const CompactFilter& CF, Acc&& accept_extra, Pre&& pre, F&& fn) {
     const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
-    const char* __restrict__ cb = reinterpret_cast<const char*>(CF.cq);
     const int xg = (int)truncf((ax + G.off0) / G.gs);
     const int yg = (int)truncf((ay + G.off1) / G.gs);
     const int zg = (int)truncf((az + G.off2) / G.gs);
     const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
     const bool replay = mode == MC_REPLAY && M.ok[i] != 0u;
     const bool record = mode == MC_RECORD;
-    auto keep = [&](const float4& q) {
-        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
-        return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) <= s_relax;
-    };
     for (uint32_t rz = 0; rz < 3u; rz++) {
         const int z = zg - 1 + (int)rz;
         if (z < 0 || z >= G.gz) continue;
         for (uint32_t ry = 0; ry < 3u; ry++) {
             const int y = yg - 1 + (int)ry;
             if (y < 0 || y >= G.gy) continue;
-            uint32_t* __restrict__ row = M.mask + (3u * rz + ry) * M.stride;
             const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
-            const uint32_t b = lb[base + (uint32_t)x0];
-            const uint32_t e = lb[base + (uint32_t)x1 + 1u];
-            uint32_t w0 = b;
-            if (replay) {
-                visit_window<VB>(xb, b, row[i], px, py, pz, s_max, accept_extra, pre, fn);
-                w0 += 32u;
-            }
-            uint32_t first = 0u;
-            for (; w0 < e; w0 += 32u) {
-                const uint32_t n = min(32u, e - w0);
-                uint32_t mask = 0u;
-                if (cb) {
-                    for (uint32_t u = 0; u < n; u += 8u) {
-                        const char* p8 = cb + ((w0 + u) << 3);
-                        uint2 e[8];
-#pragma unroll
-                        for (int t = 0; t < 8; t++) e[t] = *reinterpret_cast<const uint2*>(p8 + 8 * t);
-                        uint32_t m8 = 0u;
-#pragma unroll
-                        for (int t = 7; t >= 0; t--)
-                            m8 = __builtin_amdgcn_alignbit(m8, compact_diff(e[t], CF.qi, CF.bias), 31u);
-                        mask |= m8 << u;
-                    }
-                } else {
-                    for (uint32_t u = 0; u < n; u += 4u) {
-                        const char* p8 = xb + ((w0 + u) << 4);
-                        float4 q[4];
-#pragma unroll
-                        for (int t = 0; t < 4; t++) q[t] = *reinterpret_cast<const float4*>(p8 + 16 * t);
-                        uint32_t m8 = 0u;
-#pragma unroll
-                        for (int t = 0; t < 4; t++) m8 |= (keep(q[t]) ? 1u : 0u) << t;
-                        mask |= m8 << u;
-                    }
-                }
-                mask &= n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u;
-                if (w0 == b) first = mask;
-                visit_window<VB>(xb, w0, mask, px, py, pz, s_max, accept_extra, pre, fn);
-            }
-            if (record) row[i] = first;
+            visit_row<VB>(xb, CF, lb[base + (uint32_t)x0], lb[base + (uint32_t)x1 + 1u],
+                          M.mask + (3u * rz + ry) * M.stride, i, replay, record, px, py, pz, s_max, s_relax,
+                          accept_extra, pre, fn);
         }
     }
     if (record) M.ok[i] = 1u;
+}
+
+// The replaying walk with the three rows of a z-plane merged: each lane visits
+// its recorded kept candidates of rows y-1, y, y+1 as one sequence, VB per
+// trip, so a plane costs the wave the trips of its fullest lane instead of the
+// sum over its rows of each row's fullest lane.  A trip's gathers then come
+// from up to three row ranges of the plane.  Accepted set and visiting order
+// are those of for_neighbors_masked in MC_REPLAY mode; a plane where some lane
+// of the wave has no recorded masks or a row longer than one window of 32
+// candidates is walked row by row as there.
+template <int VB, class Acc, class Pre, class F>
+__device__ __forceinline__ void for_neighbors_planes(const GridView& G, const uint32_t* __restrict__ lb, float ax,
+                                                     float ay, float az, const float4* __restrict__ xs, float px,
+                                                     float py, float pz, float s_max, float s_relax,
+                                                     const lps_dev::NbMasks& M, uint32_t i, Acc&& accept_extra,
+                                                     Pre&& pre, F&& fn) {
+    const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
+    const int xg = (int)truncf((ax + G.off0) / G.gs);
+    const int yg = (int)truncf((ay + G.off1) / G.gs);
+    const int zg = (int)truncf((az + G.off2) / G.gs);
+    const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
+    const bool replay = M.ok[i] != 0u;
+    for (uint32_t rz = 0; rz < 3u; rz++) {
+        const int z = zg - 1 + (int)rz;
+        if (z < 0 || z >= G.gz) continue;
+        uint32_t b[3], m[3];
+        bool merge = replay;
+#pragma unroll
+        for (uint32_t ry = 0; ry < 3u; ry++) {
+            const int y = yg - 1 + (int)ry;
+            b[ry] = m[ry] = 0u;
+            if (y >= 0 && y < G.gy) {
+                const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+                b[ry] = lb[base + (uint32_t)x0];
+                const uint32_t n = lb[base + (uint32_t)x1 + 1u] - b[ry];
+                if (n > 32u) merge = false;
+                else if (n != 0u && replay) m[ry] = M.mask[(3u * rz + ry) * M.stride + i];
+            }
+        }
+        // one choice for the lanes in this plane
+        if (__all(merge)) {
+            uint32_t m0 = m[0], m1 = m[1], m2 = m[2];
+            while (m0 | m1 | m2) {
+                uint32_t jj[VB];
+                uint32_t cnt = 0u;
+#pragma unroll
+                for (int t = 0; t < VB; t++) {
+                    // the first remaining candidate in row order (none: repeat the trip's first)
+                    const bool r0 = m0 != 0u, r1 = m1 != 0u;
+                    const uint32_t mm = r0 ? m0 : (r1 ? m1 : m2);
+                    const uint32_t bb = r0 ? b[0] : (r1 ? b[1] : b[2]);
+                    const uint32_t rest = mm & (mm - 1u);
+                    if (r0) m0 = rest;
+                    else if (r1) m1 = rest;
+                    else m2 = rest;
+                    jj[t] = (t > 0 && mm == 0u) ? jj[0] : bb + (uint32_t)__ffs((int)mm) - 1u;
+                    cnt += mm != 0u ? 1u : 0u;
+                }
+                float4 q[VB];
+                decltype(pre(0u)) pv[VB];
+#pragma unroll
+                for (int t = 0; t < VB; t++) {
+                    q[t] = *reinterpret_cast<const float4*>(xb + (jj[t] << 4));
+                    pv[t] = pre(jj[t]);
+                }
+#pragma unroll
+                for (int t = 0; t < VB; t++) {
+                    const uint32_t j = jj[t];
+                    const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
+                    const float s2 = dx * dx + dy * dy + dz * dz;
+                    if ((uint32_t)t < cnt && s2 <= s_max && accept_extra(j))
+                        fn(j, q[t], s2, pv[t], make_float3(dx, dy, dz));
+                }
+            }
+        } else {
+            for (uint32_t ry = 0; ry < 3u; ry++) {
+                const int y = yg - 1 + (int)ry;
+                if (y < 0 || y >= G.gy) continue;
+                const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+                visit_row<VB>(xb, CompactFilter{}, lb[base + (uint32_t)x0], lb[base + (uint32_t)x1 + 1u],
+                              M.mask + (3u * rz + ry) * M.stride, i, replay, false, px, py, pz, s_max, s_relax,
+                              accept_extra, pre, fn);
+            }
+        }
+    }
 }
 
 // Lookup index min(trunc(sqrtf(s2) / div), 1000) of a squared distance
@@ -380,6 +473,17 @@ __device__ __forceinline__ uint32_t table_index_d(float s2, float d, const float
     const float qa = d * inv_div - 1e-3f;
     const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 1));
     return g + (s2 >= thr[g + 1u] ? 1u : 0u);
+}
+
+// The index trunc(d / div) itself from d = sqrtf(s2), by the shared-reciprocal
+// quotient of lps_div.h with inv_div = RN(1 / div): d is 0 or at least 2^-75
+// (the square root of a positive float), so the quotient is the IEEE one when
+// div lies in [2^-78, 2^40) (index_q_ok, uniform), and d < h keeps the index
+// at or below 1000.  Four instructions and no threshold read.
+__device__ __forceinline__ bool index_q_ok(float div) { return div >= 0x1p-78f && div < 0x1p40f; }
+
+__device__ __forceinline__ uint32_t table_index_q(float d, float div, float inv_div) {
+    return (uint32_t)lps_div::div_rn(d, div, inv_div);
 }
 
 // Correctly rounded square root (sqrtf).  For s >= 2^-96 the hardware
@@ -1177,8 +1281,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces(
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
     float nb = 0.f, dv = 0.f;  // warm-start features: neighbour count, velocity divergence
     // the density pass recorded this stencil's masks at the same positions
-    for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                         MC_REPLAY, CompactFilter{}, [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
+    for_neighbors_planes<2>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                         [](uint32_t) { return true; }, [&](uint32_t j) { return vel[j]; },
                          [&](uint32_t j, const float4& xj, float s2, const float4& vj, const float3&) {
         const uint32_t idx = table_index(s2, sT, c.inv_div);
         const float rj = xj.w;
@@ -1275,18 +1379,25 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const float tf = q[i];
     const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
-    for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                         MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
-                         [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
-        const float d = sqrt_rn(s2);
-        float ux, uy, uz;
-        pair_dir<false>(dl, d, i, j, fastdiv, ux, uy, uz);
-        const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
-        const float sc = tf + qj;
-        pfx += ux * sc * w;
-        pfy += uy * sc * w;
-        pfz += uz * sc * w;
-    });
+    auto pressure = [&](auto qidx) {
+        for_neighbors_planes<2>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                             [&](uint32_t j) { return i != j; }, [&](uint32_t j) { return q[j]; },
+                             [&](uint32_t j, const float4&, float s2, float qj, const float3& dl) {
+            const float d = sqrt_rn(s2);
+            float ux, uy, uz;
+            pair_dir<false>(dl, d, i, j, fastdiv, ux, uy, uz);
+            const float w = sP1[decltype(qidx)::value ? table_index_q(d, c.div, c.inv_div)
+                                                      : table_index_d(s2, d, sT, c.inv_div)];
+            const float sc = tf + qj;
+            pfx += ux * sc * w;
+            pfy += uy * sc * w;
+            pfz += uz * sc * w;
+        });
+    };
+    if (index_q_ok(c.div))
+        pressure(std::true_type{});
+    else
+        pressure(std::false_type{});
     float4 f = force[i];
     f.x += (-pfx) * c.mass * xi.w;
     f.y += (-pfy) * c.mass * xi.w;
@@ -1520,9 +1631,12 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
 }
 
 // pciCalcPressureForce (SPHKernels.cl:806-878) followed by the next
-// prediction (pciIntegrate + pciHandleCollisions) from f + fp.
+// prediction (pciIntegrate + pciHandleCollisions) from f + fp.  Held to eight
+// waves per SIMD (64 VGPRs): the compiler then spills a few values that live
+// across the whole stencil, outside its loops, and the pass runs 61.7 µs per
+// launch at 524k particles against 68.7 µs at the seven waves it would pick.
 template <int WG>
-__global__ __launch_bounds__(WG) void k_pci_pressure(
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_pci_pressure(
     const float4* __restrict__ pos, const float4* __restrict__ xq, const float* __restrict__ rho_pred, DevGrid* g,
     const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
@@ -1555,28 +1669,29 @@ __global__ __launch_bounds__(WG) void k_pci_pressure(
     const float4 xi = xq[i];
     const float tf = xi.w;
     // accepted pairs have d < 2^40; the predictions' coordinates were checked
-    // when they were written (one instantiation of the stencil per case, so
-    // the pair loop carries no mode test)
+    // when they were written; the common case also takes the table index as
+    // the quotient d / div (one instantiation of the stencil per case, so the
+    // pair loop carries no mode test)
     const bool fastdiv = c.s_max < 0x1p78f;
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     auto pressure = [&](auto nocheck) {
         // the predicted-density pass of this correction recorded the masks
-        for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xq, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
-                             MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; },
-                             [](uint32_t) { return 0; },
+        for_neighbors_planes<2>(G, cell_lb, xa.x, xa.y, xa.z, xq, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                             [&](uint32_t j) { return i != j; }, [](uint32_t) { return 0; },
                              [&](uint32_t j, const float4& xqj, float s2, int, const float3& dl) {
             const float qj = xqj.w;
             const float d = sqrt_rn(s2);
             float ux, uy, uz;
             pair_dir<decltype(nocheck)::value>(dl, d, i, j, fastdiv, ux, uy, uz);
-            const float w = sP1[table_index_d(s2, d, sT, c.inv_div)];
+            const float w = sP1[decltype(nocheck)::value ? table_index_q(d, c.div, c.inv_div)
+                                                         : table_index_d(s2, d, sT, c.inv_div)];
             const float sc = tf + qj;
             pfx += ux * sc * w;
             pfy += uy * sc * w;
             pfz += uz * sc * w;
         });
     };
-    if (fastdiv && g->tiny_pred == 0u)
+    if (fastdiv && g->tiny_pred == 0u && index_q_ok(c.div))
         pressure(std::true_type{});
     else
         pressure(std::false_type{});

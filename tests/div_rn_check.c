@@ -10,10 +10,17 @@
  * (a and d in [1, 2)) and checks the Markstein correction built on that
  * reciprocal against the IEEE quotient; the device program div_rn_check.hip
  * covers every such pair.  The uncorrected product a * (1/d) must miss some of
- * the random quotients (the check can fail).  Exit status 0 when no case fails. */
+ * the random quotients (the check can fail).  Exit status 0 when no case fails.
+ *
+ * With arguments DIV... (table steps, e.g. 0x1.a36e2ep-17) it checks instead
+ * the pressure passes' table index (table_index_q in lps_kernels.hip): for
+ * every float d in [0, 1001 * DIV), the truncated corrected quotient
+ * trunc(markstein(d, DIV, 1.0f / DIV)) equals trunc(d / DIV); the uncorrected
+ * trunc(d * (1.0f / DIV)) must miss some (the check can fail). */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 static float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
@@ -37,7 +44,27 @@ static uint64_t splitmix(uint64_t* s) {
     return z ^ (z >> 31);
 }
 
-int main(void) {
+static int check_index(float div) {
+    const float y = 1.f / div, dmax = 1001.f * div;
+    const long u1 = f2u(dmax);
+    long bad = 0, plain = 0;
+#pragma omp parallel for reduction(+ : bad, plain) schedule(static)
+    for (long u = 0; u < u1; u++) {
+        const float d = u2f((uint32_t)u);
+        const uint32_t k = (uint32_t)(d / div);
+        if ((uint32_t)markstein(d, div, y) != k) bad++;
+        if ((uint32_t)(d * y) != k) plain++;
+    }
+    printf("div %a: %ld distances checked, %ld wrong (%ld wrong uncorrected)\n", div, u1, bad, plain);
+    return bad != 0 || plain == 0;
+}
+
+int main(int argc, char** argv) {
+    if (argc > 1) {
+        int rc = 0;
+        for (int k = 1; k < argc; k++) rc |= check_index(strtof(argv[k], NULL));
+        return rc;
+    }
     const long u0 = f2u(0x1p-78f), u1 = f2u(0x1p40f);
     long bad = 0, n = 0, ones = 0;
 #pragma omp parallel for reduction(+ : bad, n, ones) schedule(static)

@@ -54,3 +54,23 @@ def test_device_reciprocal_matches_ieee(gpu):
 def test_device_quotients_match_ieee(gpu, a0, a1):
     r = _run("pairs", str(a0), str(a1))
     assert r.returncode == 0 and " 0 wrong" in r.stdout
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc not available")
+def test_table_index_quotient_is_exact_cpu(tmp_path, oracle):
+    """The pressure passes take the lookup index trunc(d / div) as the
+    corrected quotient with div's correctly rounded reciprocal (table_index_q).
+    For the table step of every scene the tests and the benchmark run, every
+    float distance below 1001 steps gives the IEEE quotient's index."""
+    from liquidsimulation_amd import scenarios, scenes
+    params = [scenes.small_block().params, scenes.sph_dam_break_16k().params,
+              scenes.pcisph_dam_break_524k().params, scenes.pcisph_obstacles_2m().params,
+              scenarios.viewer_params()]
+    divs = sorted({oracle.derived(p)[1] for p in params})
+    exe = str(tmp_path / "div_rn_check")
+    subprocess.run(["gcc", "-O2", "-fopenmp", "-mfma", "-ffp-contract=off", "-o", exe,
+                    os.path.join(HERE, "div_rn_check.c"), "-lm"], check=True)
+    r = subprocess.run([exe, *[float(d).hex() for d in divs]], capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.count(" 0 wrong") == len(divs)
