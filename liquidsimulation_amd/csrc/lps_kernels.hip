@@ -258,6 +258,32 @@ __device__ __forceinline__ uint32_t compact_diff(const uint2& e, const uint2& qi
 // whole kernel.  A row holds about four kept candidates, so VB = 3 or 4
 // mostly fills slots with repeats, and VB = 1 leaves one load in flight;
 // both measured slower at 524k particles.
+// A visitor called for every slot of a trip with whether the slot is an
+// accepted pair, for branch-free bodies.  The density sums add +0 for the
+// other slots, which leaves a sum of non-negative terms (never -0) unchanged,
+// so the result is that of the branching visit.
+template <class F>
+struct Predicated {
+    F f;
+};
+template <class T>
+struct is_predicated : std::false_type {};
+template <class F>
+struct is_predicated<Predicated<F>> : std::true_type {};
+template <class F>
+__device__ __forceinline__ Predicated<F> predicated(F f) {
+    return Predicated<F>{f};
+}
+
+template <class F, class P>
+__device__ __forceinline__ void visit_slot(F& fn, bool ok, uint32_t j, const float4& q, float s2, const P& pv,
+                                           const float3& dl) {
+    if constexpr (is_predicated<std::decay_t<F>>::value)
+        fn.f(ok, j, q, s2, pv, dl);
+    else if (ok)
+        fn(j, q, s2, pv, dl);
+}
+
 template <int VB, class Acc, class Pre, class F>
 __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px,
                                              float py, float pz, float s_max, Acc& accept_extra, Pre& pre, F& fn) {
@@ -282,6 +308,8 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
             q[t] = *reinterpret_cast<const float4*>(xb + ((w0 + kk[t]) << 4));
             pv[t] = pre(w0 + kk[t]);
         }
+        // every load of the trip issues before the first use of any
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 0; t < VB; t++) {
             // slots past cnt repeat candidate kk[0]; one predicate keeps the
@@ -289,7 +317,8 @@ __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32
             const uint32_t j = w0 + kk[t];
             const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
             const float s2 = dx * dx + dy * dy + dz * dz;
-            if ((uint32_t)t < cnt && s2 <= s_max && accept_extra(j)) fn(j, q[t], s2, pv[t], make_float3(dx, dy, dz));
+            visit_slot(fn, (uint32_t)t < cnt && s2 <= s_max && accept_extra(j), j, q[t], s2, pv[t],
+                       make_float3(dx, dy, dz));
         }
     }
 }
@@ -441,8 +470,8 @@ __device__ __forceinline__ void for_neighbors_planes(const GridView& G, const ui
                     const uint32_t j = jj[t];
                     const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
                     const float s2 = dx * dx + dy * dy + dz * dz;
-                    if ((uint32_t)t < cnt && s2 <= s_max && accept_extra(j))
-                        fn(j, q[t], s2, pv[t], make_float3(dx, dy, dz));
+                    visit_slot(fn, (uint32_t)t < cnt && s2 <= s_max && accept_extra(j), j, q[t], s2, pv[t],
+                               make_float3(dx, dy, dz));
                 }
             }
         } else {
@@ -461,17 +490,19 @@ __device__ __forceinline__ void for_neighbors_planes(const GridView& G, const ui
 // Lookup index min(trunc(sqrtf(s2) / div), 1000) of a squared distance
 // s2 <= s_max from the threshold table (see Derived::thr): the hardware
 // square root and 1/div give the quotient to within 5e-4; one below that,
-// truncated, is the exact index or one less, and the threshold decides.
+// truncated, is the exact index or one less, and the threshold decides.  The
+// guess is held below 1000, so the index stays in the table for every s2
+// (NaN and +inf included), which the branch-free density visits rely on.
 __device__ __forceinline__ uint32_t table_index(float s2, const float* __restrict__ thr, float inv_div) {
     const float qa = __builtin_amdgcn_sqrtf(s2) * inv_div - 1e-3f;
-    const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 1));
+    const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 2));
     return g + (s2 >= thr[g + 1u] ? 1u : 0u);
 }
 
 // The same index when d = sqrtf(s2) is already at hand (a closer first guess).
 __device__ __forceinline__ uint32_t table_index_d(float s2, float d, const float* __restrict__ thr, float inv_div) {
     const float qa = d * inv_div - 1e-3f;
-    const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 1));
+    const uint32_t g = min((uint32_t)fmaxf(qa, 0.f), (uint32_t)(LPS_TABLE_SIZE - 2));
     return g + (s2 >= thr[g + 1u] ? 1u : 0u);
 }
 
@@ -1237,7 +1268,10 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
     for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                          cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
-                         [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
+                         predicated([&](bool ok, uint32_t, const float4&, float s2, int, const float3&) {
+                             const float w = sW[table_index(s2, sT, c.inv_div)];
+                             sum += ok ? w : 0.f;
+                         }));
     const float rho = c.mass * sum;
     float p = c.stiffness * (rho - c.rest_density);
     if (p < 0.f) p *= c.npf;
@@ -1445,7 +1479,10 @@ __global__ __launch_bounds__(WG) void k_pci_density(
         // recorded for the pressure pass of this correction (same positions and cells)
         for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
-                             [&](uint32_t, const float4&, float s2, int, const float3&) { sum += sW[table_index(s2, sT, c.inv_div)]; });
+                             predicated([&](bool ok, uint32_t, const float4&, float s2, int, const float3&) {
+                                 const float w = sW[table_index(s2, sT, c.inv_div)];
+                                 sum += ok ? w : 0.f;
+                             }));
         const float rp = c.mass * sum;
         float e = rp - c.rest_density;
         float dp = c.delta * e;
@@ -1636,7 +1673,7 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
 // across the whole stencil, outside its loops, and the pass runs 61.7 µs per
 // launch at 524k particles against 68.7 µs at the seven waves it would pick.
 template <int WG>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(8))) void k_pci_pressure(
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 8 : 1))) void k_pci_pressure(
     const float4* __restrict__ pos, const float4* __restrict__ xq, const float* __restrict__ rho_pred, DevGrid* g,
     const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, float4* __restrict__ fp, const float4* __restrict__ force,
