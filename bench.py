@@ -98,7 +98,8 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(
         s.update(scene.dt, warmup)
     if fixed_tuning is not None:
         s.set_tuning(*fixed_tuning)
-        tuning = dict(zip(("use_graphs", "lookahead", "correction_threads"), fixed_tuning), fixed=True)
+        tuning = dict(zip(("use_graphs", "lookahead", "correction_threads", "lanes_per_particle"), fixed_tuning),
+                      fixed=True)
     else:
         tuning = s.autotune(scene.dt, 10) if scene.params.method == 1 else None
     stream = torch.cuda.ExternalStream(s.stream())
@@ -134,7 +135,8 @@ def run_gpu(scene, steps: int, warmup: int, profile: bool, weights=None, marks=(
         if warmup:
             s.update(scene.dt, warmup)
         if tuning:
-            s.set_tuning(tuning["use_graphs"], tuning["lookahead"], tuning["correction_threads"])
+            s.set_tuning(tuning["use_graphs"], tuning["lookahead"], tuning["correction_threads"],
+                         tuning.get("lanes_per_particle", 0))
         ke, pe = s.get_energy()
         e_samples.append(float(ke.sum() + pe.sum()))
         s.profile(True)
@@ -421,8 +423,8 @@ def main():
                     help="timed CPU steps (default: the GPU's timed steps, at most 40)")
     ap.add_argument("--mode", choices=("both", "plain"), default="both")
     ap.add_argument("--tuning", default=None,
-                    help="graphs,lookahead,threads (e.g. 0,2,256): fixed launch settings, no autotuning "
-                         "(counter runs)")
+                    help="graphs,lookahead,threads[,lanes] (e.g. 0,2,256): fixed launch settings, no "
+                         "autotuning (counter runs); lanes per particle 0 (automatic), 1 or 4")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy five times (scales the HBM byte counters)")
     args = ap.parse_args()
@@ -443,8 +445,8 @@ def main():
     scene = scenes.WORKLOADS[args.workload]()
     fixed = None
     if args.tuning:
-        g_, l_, t_ = (int(v) for v in args.tuning.split(","))
-        fixed = (bool(g_), l_, t_)
+        vals = [int(v) for v in args.tuning.split(",")]
+        fixed = (bool(vals[0]), vals[1], vals[2], vals[3] if len(vals) > 3 else 0)
     cpu_steps = args.cpu_steps if args.cpu_steps > 0 else min(args.steps, 40)
     marks = (cpu_steps, 230)
     summaries = {"learned": None, "learned_always": None}

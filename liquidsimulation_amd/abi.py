@@ -110,6 +110,7 @@ class CTuning(C.Structure):
         ("lookahead", C.c_uint32),
         ("ms_per_step", C.c_float),
         ("correction_threads", C.c_uint32),
+        ("lanes_per_particle", C.c_uint32),
     ]
 
 

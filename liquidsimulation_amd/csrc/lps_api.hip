@@ -213,6 +213,7 @@ struct lps_ctx {
     bool graphs_on = true;
     uint32_t lookahead = 1;   // corrections kept queued ahead of the one waited for
     uint32_t corr_threads = LPS_BLOCK;  // workgroup size of the correction kernels
+    uint32_t lanes = 0;                 // lanes per particle in the stencil kernels: 0 automatic, 1 or 4
 
     // profiling
     bool prof = false;
@@ -466,6 +467,11 @@ lps_dev::Scenes scenes(const lps_ctx* c) {
                            c->d_actlist};
 }
 
+// Whether the stencil kernels give each particle a quad of lanes.  By
+// default below 512 particle blocks (131,072 slots): one lane per particle
+// then fills at most two waves per SIMD.
+bool quad_lanes(const lps_ctx* c) { return c->lanes == 4u || (c->lanes == 0u && c->n / LPS_BLOCK < 512u); }
+
 lps_dev::PhysArgs phys_args(lps_ctx* c, const SimConst& k) {
     lps_dev::PhysArgs a;
     a.sc = scenes(c);
@@ -487,6 +493,7 @@ lps_dev::PhysArgs phys_args(lps_ctx* c, const SimConst& k) {
     a.mk.stride = c->cap_n;
     a.mk.cq = c->mk_cq;
     a.c = k;
+    a.quad = quad_lanes(c) ? 1u : 0u;
     return a;
 }
 
@@ -1415,19 +1422,26 @@ int lps_get_energy(lps_ctx* c, double* kinetic, double* potential, uint32_t nsce
 }
 
 bool valid_corr_threads(uint32_t t) { return t == 64u || t == 128u || t == (uint32_t)LPS_BLOCK; }
+bool valid_lanes(uint32_t l) { return l == 0u || l == 1u || l == 4u; }
 
 void set_corr_threads(lps_ctx* c, uint32_t t) {
     if (c->corr_threads != t) c->epoch++;  // captured prefixes hold the old launch shape
     c->corr_threads = t;
 }
 
+void set_lanes(lps_ctx* c, uint32_t l) {
+    if (c->lanes != l) c->epoch++;
+    c->lanes = l;
+}
+
 int lps_set_tuning(lps_ctx* c, const lps_tuning* t) {
     if (!c || !t || (t->use_graphs != 0 && t->use_graphs != 1) || t->lookahead < 1 || t->lookahead > 4 ||
-        !valid_corr_threads(t->correction_threads))
+        !valid_corr_threads(t->correction_threads) || !valid_lanes(t->lanes_per_particle))
         return LPS_ERR_INVALID;
     c->graphs_on = t->use_graphs != 0;
     c->lookahead = t->lookahead;
     set_corr_threads(c, t->correction_threads);
+    set_lanes(c, t->lanes_per_particle);
     return LPS_OK;
 }
 
@@ -1437,6 +1451,7 @@ int lps_get_tuning(lps_ctx* c, lps_tuning* t) {
     t->lookahead = c->lookahead;
     t->ms_per_step = 0.f;
     t->correction_threads = c->corr_threads;
+    t->lanes_per_particle = quad_lanes(c) ? 4u : 1u;
     return LPS_OK;
 }
 
@@ -1479,19 +1494,23 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
         return e;
     };
     const bool g0 = c->graphs_on;
-    const uint32_t l0 = c->lookahead, t0 = c->corr_threads;
+    const uint32_t l0 = c->lookahead, t0 = c->corr_threads, q0 = c->lanes;
     const uint32_t coll0 = c->coll_stride;
     c->coll_stride = 0;  // no training rows from trial steps
     const WarmFallback wf0 = c->wf;
     const uint64_t steps0 = c->steps;
     if (copy_all(true) != hipSuccess) st = LPS_ERR_HIP;
     // graphs x lookahead x correction-kernel workgroup size (the particle
-    // block of 256 slots run as 1, 2 or 4 workgroups)
-    struct Cand { bool g; uint32_t l, t; float ms; };
+    // block of 256 slots run as 1, 2 or 4 workgroups) with one lane per
+    // particle; below 2048 particle blocks also graphs x lookahead with a
+    // quad per particle (where the workgroup size does not apply)
+    struct Cand { bool g; uint32_t l, t, q; float ms; };
     std::vector<Cand> cands;
     for (int g = 0; g < 2; g++)
-        for (uint32_t l = 1; l <= 3; l++)
-            for (uint32_t t : {(uint32_t)LPS_BLOCK, 128u, 64u}) cands.push_back({g == 1, l, t, 0.f});
+        for (uint32_t l = 1; l <= 3; l++) {
+            for (uint32_t t : {(uint32_t)LPS_BLOCK, 128u, 64u}) cands.push_back({g == 1, l, t, 1u, 0.f});
+            if (n / LPS_BLOCK < 2048u) cands.push_back({g == 1, l, t0, 4u, 0.f});
+        }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = LPS_ERR_HIP;
     for (Cand& cd : cands) {
@@ -1499,6 +1518,7 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
         c->graphs_on = cd.g;
         c->lookahead = cd.l;
         set_corr_threads(c, cd.t);
+        set_lanes(c, cd.q);
         // one untimed step (graph capture), then the timed ones, each from the snapshot
         for (int pass = 0; pass < 2 && !st; pass++) {
             c->cur = (int)cur;
@@ -1532,6 +1552,7 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
         c->graphs_on = g0;
         c->lookahead = l0;
         set_corr_threads(c, t0);
+        set_lanes(c, q0);
         return st;
     }
     const Cand* best = &cands[0];
@@ -1540,11 +1561,13 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
     c->graphs_on = best->g;
     c->lookahead = best->l;
     set_corr_threads(c, best->t);
+    set_lanes(c, best->q);
     if (chosen) {
         chosen->use_graphs = best->g ? 1 : 0;
         chosen->lookahead = best->l;
         chosen->ms_per_step = best->ms;
         chosen->correction_threads = best->t;
+        chosen->lanes_per_particle = best->q;
     }
     return LPS_OK;
 }

@@ -76,6 +76,7 @@ struct PhysArgs {
     const lps_plane* planes;
     NbMasks mk;
     SimConst c;
+    uint32_t quad;        // 1: density, forces and SPH pressure give each particle a quad of lanes
 };
 
 // per-block partial records (bounding box and energies) of a state

@@ -1439,6 +1439,359 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     force[i] = f;
 }
 
+// ===================================================== quad stencils ====
+// Small scenes leave most of the chip idle while every lane walks a long
+// neighbour list alone: the 16k SPH dam break's compressed fluid gives a
+// particle hundreds of candidates, and its 63 particle blocks fill a quarter
+// of the SIMDs with one wave each.  The quad kernels give a particle the four
+// lanes of a quad.  Pass 1 tests every fourth candidate of a window on each
+// lane, and the quad ORs the four masks.  Pass 2 hands the quad's next four
+// kept candidates to its four lanes in order; each lane computes the terms
+// of its candidate, and every lane of the quad then adds the four terms in
+// candidate order (DPP quad broadcasts), so each accumulator takes exactly the
+// additions of the one-lane walk.  A rejected candidate contributes +0: the
+// accumulators start at +0, a float sum is -0 only when both operands are -0,
+// so they never hold -0 and adding +0 leaves them unchanged.  The four lanes
+// of a quad share one particle, so they are active together throughout.
+constexpr uint32_t LPS_QUAD = 4u;
+
+__device__ __forceinline__ uint32_t quad_sub() { return threadIdx.x & (LPS_QUAD - 1u); }
+
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), K * 0x55, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ uint32_t quad_or(uint32_t m) {
+    m |= (uint32_t)__builtin_amdgcn_mov_dpp((int)m, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    m |= (uint32_t)__builtin_amdgcn_mov_dpp((int)m, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    return m;
+}
+
+// Pass 1 over one window of n <= 32 candidates from w0, candidates sub,
+// sub + 4, ... on lane sub; the quad's accept mask on every lane.
+__device__ __forceinline__ uint32_t quad_filter(const char* __restrict__ xb, const CompactFilter CF, uint32_t w0,
+                                                uint32_t n, float px, float py, float pz, float s_relax) {
+    const uint32_t sub = quad_sub();
+    uint32_t bits = 0u;
+    if (CF.cq) {
+        const char* __restrict__ cb = reinterpret_cast<const char*>(CF.cq);
+        uint2 e[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t u = sub + LPS_QUAD * (uint32_t)t;
+            e[t] = CF.qi;
+            if (u < n) e[t] = *reinterpret_cast<const uint2*>(cb + ((w0 + u) << 3));
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) bits |= (compact_diff(e[t], CF.qi, CF.bias) >> 31) << (sub + LPS_QUAD * t);
+    } else {
+        float4 q[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const uint32_t u = sub + LPS_QUAD * (uint32_t)t;
+            q[t] = u < n ? *reinterpret_cast<const float4*>(xb + ((w0 + u) << 4)) : make_float4(px, py, pz, 0.f);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const float dx = q[t].x - px, dy = q[t].y - py, dz = q[t].z - pz;
+            bits |= (__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)) <= s_relax ? 1u : 0u)
+                    << (sub + LPS_QUAD * t);
+        }
+    }
+    return quad_or(bits) & (n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u);
+}
+
+// Pass 2 of one window for a quad: `term(ok, j, x_j, s2, pre(j), x_j - x_i, t)`
+// writes the NT terms of candidate j (zeros when not ok), and acc[a] takes
+// the terms of the quad's candidates in order.
+template <int NT, class Acc, class Pre, class Term>
+__device__ __forceinline__ void quad_visit(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px, float py,
+                                           float pz, float s_max, Acc& accept_extra, Pre& pre, Term& term,
+                                           float (&acc)[NT]) {
+    const uint32_t sub = quad_sub();
+    while (mask) {
+        uint32_t mm = mask;
+        if (sub > 0u) mm &= mm - 1u;
+        if (sub > 1u) mm &= mm - 1u;
+        if (sub > 2u) mm &= mm - 1u;
+        const uint32_t cnt = min((uint32_t)__popc(mask), LPS_QUAD);
+#pragma unroll
+        for (uint32_t t = 0; t < LPS_QUAD; t++) mask &= mask - 1u;
+        const bool have = mm != 0u;
+        const uint32_t j = have ? w0 + (uint32_t)__ffs((int)mm) - 1u : w0;
+        const float4 q = *reinterpret_cast<const float4*>(xb + (j << 4));
+        const auto pv = pre(j);
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        const float s2 = dx * dx + dy * dy + dz * dz;
+        float t[NT];
+        term(have && s2 <= s_max && accept_extra(j), j, q, s2, pv, make_float3(dx, dy, dz), t);
+#pragma unroll
+        for (int a = 0; a < NT; a++) acc[a] += quad_bcast<0>(t[a]);
+        if (cnt > 1u) {
+#pragma unroll
+            for (int a = 0; a < NT; a++) acc[a] += quad_bcast<1>(t[a]);
+        }
+        if (cnt > 2u) {
+#pragma unroll
+            for (int a = 0; a < NT; a++) acc[a] += quad_bcast<2>(t[a]);
+        }
+        if (cnt > 3u) {
+#pragma unroll
+            for (int a = 0; a < NT; a++) acc[a] += quad_bcast<3>(t[a]);
+        }
+    }
+}
+
+// for_neighbors_masked for a quad (same rows, windows, recorded masks and
+// visiting order; the recording lane is the quad's first).
+template <int NT, class Acc, class Pre, class Term>
+__device__ __forceinline__ void for_neighbors_quad(const GridView& G, const uint32_t* __restrict__ lb, float ax,
+                                                   float ay, float az, const float4* __restrict__ xs, float px,
+                                                   float py, float pz, float s_max, float s_relax,
+                                                   const lps_dev::NbMasks& M, uint32_t i, int mode,
+                                                   const CompactFilter& CF, Acc&& accept_extra, Pre&& pre,
+                                                   Term&& term, float (&acc)[NT]) {
+    const char* __restrict__ xb = reinterpret_cast<const char*>(xs);
+    const int xg = (int)truncf((ax + G.off0) / G.gs);
+    const int yg = (int)truncf((ay + G.off1) / G.gs);
+    const int zg = (int)truncf((az + G.off2) / G.gs);
+    const int x0 = max(xg - 1, 0), x1 = min(xg + 1, G.gx - 1);
+    const bool replay = mode == MC_REPLAY && M.ok[i] != 0u;
+    const bool record = mode == MC_RECORD && quad_sub() == 0u;
+    for (uint32_t rz = 0; rz < 3u; rz++) {
+        const int z = zg - 1 + (int)rz;
+        if (z < 0 || z >= G.gz) continue;
+        for (uint32_t ry = 0; ry < 3u; ry++) {
+            const int y = yg - 1 + (int)ry;
+            if (y < 0 || y >= G.gy) continue;
+            uint32_t* __restrict__ row = M.mask + (3u * rz + ry) * M.stride;
+            const uint32_t base = G.base + (uint32_t)G.gx * (uint32_t)y + G.gxy * (uint32_t)z;
+            const uint32_t b = lb[base + (uint32_t)x0];
+            const uint32_t e = lb[base + (uint32_t)x1 + 1u];
+            uint32_t w0 = b;
+            if (replay) {
+                quad_visit<NT>(xb, b, row[i], px, py, pz, s_max, accept_extra, pre, term, acc);
+                w0 += 32u;
+            }
+            uint32_t first = 0u;
+            for (; w0 < e; w0 += 32u) {
+                const uint32_t mask = quad_filter(xb, CF, w0, min(32u, e - w0), px, py, pz, s_relax);
+                if (w0 == b) first = mask;
+                quad_visit<NT>(xb, w0, mask, px, py, pz, s_max, accept_extra, pre, term, acc);
+            }
+            if (record) row[i] = first;
+        }
+    }
+    if (record) M.ok[i] = 1u;
+}
+
+// The particle of a quad kernel's thread: launch block lb covers LPS_BLOCK /
+// 4 particles, four lanes each.
+__device__ __forceinline__ uint32_t quad_particle(uint32_t lb) {
+    return lb * (LPS_BLOCK / LPS_QUAD) + threadIdx.x / LPS_QUAD;
+}
+
+// k_density with a quad per particle.
+__global__ __launch_bounds__(LPS_BLOCK) void k_density_q(const float4* __restrict__ pos, const DevGrid* g,
+                                                         lps_dev::Scenes S, const uint32_t* __restrict__ cell_lb,
+                                                         SimConst c, const float* __restrict__ tables,
+                                                         float4* __restrict__ posrho, float* __restrict__ press,
+                                                         float* __restrict__ q, lps_dev::NbMasks mk,
+                                                         uint32_t nlogical) {
+    if (step_dead(g)) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
+    load_table(sW, tables);
+    load_thresholds(sT, tables);
+    __syncthreads();
+    const uint32_t i = quad_particle(lb);
+    const BlockScene B = block_scene(S, i / LPS_BLOCK);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
+    const float4 xi = pos[i];
+    const float qs = compact_scale(G);
+    CompactFilter cf{};
+    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
+    float acc[1] = {0.f};
+    for_neighbors_quad<1>(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
+                          cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+                          [&](bool ok, uint32_t, const float4&, float s2, int, const float3&, float (&t)[1]) {
+                              t[0] = ok ? sW[table_index(s2, sT, c.inv_div)] : 0.f;
+                          },
+                          acc);
+    if (quad_sub() != 0u) return;
+    const float rho = c.mass * acc[0];
+    float p = c.stiffness * (rho - c.rest_density);
+    if (p < 0.f) p *= c.npf;
+    posrho[i] = make_float4(xi.x, xi.y, xi.z, rho);
+    press[i] = p;
+    q[i] = p / (rho * rho);
+}
+
+// k_forces with a quad per particle.
+template <bool FEAT>
+__global__ __launch_bounds__(LPS_BLOCK) void k_forces_q(
+    const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
+    const uint8_t* __restrict__ first, DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
+    const float* __restrict__ tables, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
+    const lps_plane* __restrict__ planes0, float4* __restrict__ force, float4* __restrict__ xpred,
+    const float* __restrict__ pprev, float* __restrict__ feat, lps_dev::Scenes S, lps_dev::NbMasks mk,
+    uint32_t nlogical) {
+    if (step_dead(g)) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
+    load_table(sW1, tables + 1 * LPS_TABLE_SIZE);
+    load_table(sW2, tables + 2 * LPS_TABLE_SIZE);
+    load_table(sV2, tables + 4 * LPS_TABLE_SIZE);
+    load_thresholds(sT, tables);
+    __syncthreads();
+    const uint32_t i = quad_particle(lb);
+    const BlockScene B = block_scene(S, i / LPS_BLOCK);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
+    const SimConst c = scene_const(c0, B.info);
+    const float4 xi = posrho[i];
+    const float4 vi = vel[i];
+    const float ri = xi.w;
+    // snx, sny, snz, lc, vfx, vfy, vfz, then (features) nb, dv
+    constexpr int NT = FEAT ? 9 : 7;
+    float acc[NT];
+#pragma unroll
+    for (int a = 0; a < NT; a++) acc[a] = 0.f;
+    for_neighbors_quad<NT>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                           MC_REPLAY, CompactFilter{}, [](uint32_t) { return true; },
+                           [&](uint32_t j) { return vel[j]; },
+                           [&](bool ok, uint32_t j, const float4& xj, float s2, const float4& vj, const float3&,
+                               float (&t)[NT]) {
+        if (!ok) {
+#pragma unroll
+            for (int a = 0; a < NT; a++) t[a] = 0.f;
+            return;
+        }
+        const uint32_t idx = table_index(s2, sT, c.inv_div);
+        const float rj = xj.w;
+        const float w1 = sW1[idx], wv = sV2[idx];
+        float n[8] = {(xi.x - xj.x) * w1, (xi.y - xj.y) * w1, (xi.z - xj.z) * w1, sW2[idx],
+                      (vj.x - vi.x) * wv, (vj.y - vi.y) * wv, (vj.z - vi.z) * wv, 0.f};
+        constexpr int NQ = FEAT ? 8 : 7;
+        if constexpr (FEAT)
+            n[7] = ((vj.x - vi.x) * (xi.x - xj.x) + (vj.y - vi.y) * (xi.y - xj.y) + (vj.z - vi.z) * (xi.z - xj.z)) * w1;
+        uint32_t key = lps_div::num_key(n[0]);
+        for (int a = 1; a < NQ; a++) key = min(key, lps_div::num_key(n[a]));
+        if (rj < 0x1p40f && lps_div::div_rn_ok(rj, key)) {
+            const float y = lps_div::rcp_rn(rj);
+            for (int a = 0; a < NQ; a++) n[a] = lps_div::div_rn(n[a], rj, y);
+        } else {
+            for (int a = 0; a < NQ; a++) n[a] = n[a] / rj;
+        }
+        const bool other = i != j;
+        t[0] = n[0];
+        t[1] = n[1];
+        t[2] = n[2];
+        t[3] = n[3];
+        t[4] = other ? n[4] : 0.f;
+        t[5] = other ? n[5] : 0.f;
+        t[6] = other ? n[6] : 0.f;
+        if constexpr (FEAT) {
+            t[7] = other ? 1.f : 0.f;
+            t[8] = other ? n[7] : 0.f;
+        }
+    }, acc);
+    if (quad_sub() != 0u) return;
+    float snx = acc[0], sny = acc[1], snz = acc[2];
+    const float lc = acc[3], vfx = acc[4], vfy = acc[5], vfz = acc[6];
+    float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
+    snx *= c.mass; sny *= c.mass; snz *= c.mass;
+    const float snl = sqrtf(snx * snx + sny * sny + snz * snz);
+    if constexpr (FEAT) {
+        float* fr = feat + i;
+        const size_t fs = mk.stride;
+        fr[0 * fs] = ri / c.rest_density - 1.f;
+        fr[1 * fs] = acc[7] * (1.f / 32.f);
+        fr[2 * fs] = acc[8] * c.mass * c.dt;
+        fr[3 * fs] = snx * c.h;
+        fr[4 * fs] = sny * c.h;
+        fr[5 * fs] = snz * c.h;
+        fr[6 * fs] = snl * c.h;
+        fr[7 * fs] = pprev[i] / (c.delta * c.rest_density);
+        fr[8 * fs] = sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * c.dt / c.h;
+        fr[9 * fs] = first[i] ? 1.f : 0.f;
+    }
+    if (snl > c.st_threshold) {
+        const float k = -c.surface_tension * lc * c.mass;
+        fx += (snx / snl) * k;
+        fy += (sny / snl) * k;
+        fz += (snz / snl) * k;
+    }
+    fx += vfx * c.viscosity * c.mass;
+    fy += vfy * c.viscosity * c.mass;
+    fz += vfz * c.viscosity * c.mass;
+    force[i] = make_float4(fx, fy, fz, 0.f);
+    if (xpred) {
+        const float4 xo = make_float4(xi.x, xi.y, xi.z, 0.f);
+        const float4 xp = predict(xo, vi, vh[i], first[i] != 0, fx + 0.f, fy + 0.f, fz + 0.f, ri, c,
+                                  boxes0 + B.info.box0, spheres0 + B.info.sphere0, planes0 + B.info.plane0);
+        xpred[i] = xp;
+        mk.cq[i] = quantize(xp.x, xp.y, xp.z, G, compact_scale(G));
+        if (lps_div::tiny_coord(xp.x, xp.y, xp.z)) g->tiny_pred = 1u;
+        if constexpr (FEAT) {
+            const float ex = xp.x - xo.x, ey = xp.y - xo.y, ez = xp.z - xo.z;
+            feat[(size_t)10 * mk.stride + i] = sqrtf(ex * ex + ey * ey + ez * ez) / c.h;
+        }
+    }
+}
+
+// k_pressure_sph with a quad per particle.
+__global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph_q(const float4* __restrict__ posrho,
+                                                              const float* __restrict__ q, const DevGrid* g,
+                                                              const uint32_t* __restrict__ cell_lb, SimConst c,
+                                                              const float* __restrict__ tables,
+                                                              float4* __restrict__ force, lps_dev::Scenes S,
+                                                              lps_dev::NbMasks mk, uint32_t nlogical) {
+    if (step_dead(g)) return;
+    const uint32_t lb = xcd_block(nlogical);
+    if (lb == UINT_MAX) return;
+    __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
+    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
+    load_thresholds(sT, tables);
+    __syncthreads();
+    const uint32_t i = quad_particle(lb);
+    const BlockScene B = block_scene(S, i / LPS_BLOCK);
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
+    const float4 xi = posrho[i];
+    const float tf = q[i];
+    const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
+    const bool qidx = index_q_ok(c.div);
+    float acc[3] = {0.f, 0.f, 0.f};
+    for_neighbors_quad<3>(G, cell_lb, xi.x, xi.y, xi.z, posrho, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                          MC_REPLAY, CompactFilter{}, [&](uint32_t j) { return i != j; },
+                          [&](uint32_t j) { return q[j]; },
+                          [&](bool ok, uint32_t j, const float4&, float s2, float qj, const float3& dl, float (&t)[3]) {
+        if (!ok) {
+            t[0] = t[1] = t[2] = 0.f;
+            return;
+        }
+        const float d = sqrt_rn(s2);
+        float ux, uy, uz;
+        pair_dir<false>(dl, d, i, j, fastdiv, ux, uy, uz);
+        const float w = sP1[qidx ? table_index_q(d, c.div, c.inv_div) : table_index_d(s2, d, sT, c.inv_div)];
+        const float sc = tf + qj;
+        t[0] = ux * sc * w;
+        t[1] = uy * sc * w;
+        t[2] = uz * sc * w;
+    }, acc);
+    if (quad_sub() != 0u) return;
+    float4 f = force[i];
+    f.x += (-acc[0]) * c.mass * xi.w;
+    f.y += (-acc[1]) * c.mass * xi.w;
+    f.z += (-acc[2]) * c.mass * xi.w;
+    force[i] = f;
+}
+
 // pciCalcDensityPressure (SPHKernels.cl:737-804) + the density-error maximum
 // of correction k (max_i |e_i|, e_i = rho*_i - rho0 scaled by npf when the
 // pressure update is negative; PCISPHSolver.cpp:113-121,152-156).  Cells come
@@ -1745,6 +2098,147 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
     if (lps_div::tiny_coord(xn.x, xn.y, xn.z)) g->tiny_pred = 1u;
 }
 
+// k_pci_density with a quad per particle: a workgroup of 256 threads takes a
+// 64-slot piece of an active particle block.
+template <bool FEAT>
+__global__ __launch_bounds__(LPS_BLOCK) void k_pci_density_q(
+    const float4* __restrict__ pos, const float4* __restrict__ xpred, DevGrid* g, const uint32_t* __restrict__ cell_lb,
+    SimConst c, const float* __restrict__ tables, float* __restrict__ rho_pred, float* __restrict__ press,
+    float4* __restrict__ xq, uint32_t k, float* __restrict__ feat, const float4* __restrict__ posrho,
+    lps_dev::Scenes S, lps_dev::NbMasks mk, uint32_t nlogical) {
+    const uint32_t slot = (k - 1u) % LPS_MAX_ITER_SLOTS;
+    if (step_dead(g) || g->running == 0u) return;
+    const uint32_t L = g->list_of[k % LPS_MAX_ITER_SLOTS];
+    uint32_t s0 = 0u;
+    const uint32_t lb = active_piece<LPS_BLOCK / LPS_QUAD>(g, S, L, s0);
+    if (lb == UINT_MAX) return;
+    __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
+    __shared__ uint32_t s_max[LPS_BLOCK / 64];
+    const BlockScene B = block_scene(S, lb);
+    load_table(sW, tables);
+    load_thresholds(sT, tables);
+    __syncthreads();
+    const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
+    uint32_t ebits = 0;
+    if (i < B.info.slot0 + B.info.count) {
+        const GridView G = grid_view(g, S.grid + B.s);
+        const float4 xa = pos[i];
+        const float4 xi = xpred[i];
+        const float qs = compact_scale(G);
+        CompactFilter cf{};
+        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
+        float acc[1] = {0.f};
+        for_neighbors_quad<1>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
+                              MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
+                              [&](bool ok, uint32_t, const float4&, float s2, int, const float3&, float (&t)[1]) {
+                                  t[0] = ok ? sW[table_index(s2, sT, c.inv_div)] : 0.f;
+                              },
+                              acc);
+        if (quad_sub() == 0u) {
+            const float rp = c.mass * acc[0];
+            float e = rp - c.rest_density;
+            float dp = c.delta * e;
+            if (dp < 0.f) {
+                dp *= c.npf;
+                e *= c.npf;
+            }
+            const SceneGrid* sg = S.grid + B.s;
+            const size_t ob = (size_t)sg->buf_of[k % LPS_MAX_ITER_SLOTS] * mk.stride;
+            const float pold =
+                k == 1u ? 0.f : press[(size_t)sg->buf_of[(k - 1u) % LPS_MAX_ITER_SLOTS] * mk.stride + i];
+            const float p = pold + dp;
+            rho_pred[ob + i] = rp;
+            press[ob + i] = p;
+            xq[i] = make_float4(xi.x, xi.y, xi.z, p / (rp * rp));
+            if (FEAT && k == 1u) {
+                float* fr = feat + i;
+                const size_t fs = mk.stride;
+                const float e1 = rp / c.rest_density - 1.f;
+                fr[11 * fs] = e1;
+                fr[12 * fs] = fmaxf(e1, 0.f);
+                fr[13 * fs] = dp / (c.delta * c.rest_density);
+                fr[14 * fs] = (rp - posrho[i].w) / c.rest_density;
+                fr[15 * fs] = e1 * e1 * 100.f;
+            }
+            ebits = __float_as_uint(fabsf(e));
+        }
+    }
+    ebits = wave_max_u32(ebits);
+    if (lane_id() == 0) s_max[threadIdx.x >> 6] = ebits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t m = 0;
+        for (uint32_t w = 0; w < LPS_BLOCK / 64; w++) m = max(m, s_max[w]);
+        if (m) atomicMax(&S.grid[B.s].err_bits[slot], m);
+    }
+}
+
+// k_pci_pressure with a quad per particle (rows walked one by one).
+__global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure_q(
+    const float4* __restrict__ pos, const float4* __restrict__ xq, const float* __restrict__ rho_pred, DevGrid* g,
+    const uint32_t* __restrict__ cell_lb, SimConst c0, const float* __restrict__ tables, float4* __restrict__ fp,
+    const float4* __restrict__ force, const float4* __restrict__ posrho, const float4* __restrict__ vel,
+    const float4* __restrict__ vh, const uint8_t* __restrict__ first, const lps_box* __restrict__ boxes0,
+    const lps_sphere* __restrict__ spheres0, const lps_plane* __restrict__ planes0, float4* __restrict__ xpred_next,
+    uint32_t k, lps_dev::Scenes S, lps_dev::NbMasks mk, LoopRule rule, uint32_t* __restrict__ host_flag,
+    uint32_t nlogical) {
+    if (blockIdx.x == 0) loop_rule_block(g, S, k, rule, c0.rest_density, host_flag);
+    if (step_dead(g)) return;
+    uint32_t s0 = 0u;
+    const uint32_t lb = active_piece<LPS_BLOCK / LPS_QUAD>(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS], s0);
+    if (lb == UINT_MAX) return;
+    const BlockScene B = block_scene(S, lb);
+    const uint32_t stop = S.grid[B.s].stop_iter;
+    if (stop != 0u && stop < k) return;
+    __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
+    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
+    load_thresholds(sT, tables);
+    __syncthreads();
+    const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
+    if (i >= B.info.slot0 + B.info.count) return;
+    const GridView G = grid_view(g, S.grid + B.s);
+    const SimConst c = scene_const(c0, B.info);
+    const float4 xa = pos[i];
+    const float4 xi = xq[i];
+    const float tf = xi.w;
+    const bool fastdiv = c.s_max < 0x1p78f;
+    const bool nocheck = fastdiv && g->tiny_pred == 0u;
+    const bool qidx = index_q_ok(c.div);
+    float acc[3] = {0.f, 0.f, 0.f};
+    for_neighbors_quad<3>(G, cell_lb, xa.x, xa.y, xa.z, xq, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_REPLAY,
+                          CompactFilter{}, [&](uint32_t j) { return i != j; }, [](uint32_t) { return 0; },
+                          [&](bool ok, uint32_t j, const float4& xqj, float s2, int, const float3& dl, float (&t)[3]) {
+        if (!ok) {
+            t[0] = t[1] = t[2] = 0.f;
+            return;
+        }
+        const float d = sqrt_rn(s2);
+        float ux, uy, uz;
+        if (nocheck)
+            pair_dir<true>(dl, d, i, j, fastdiv, ux, uy, uz);
+        else
+            pair_dir<false>(dl, d, i, j, fastdiv, ux, uy, uz);
+        const float w = sP1[nocheck && qidx ? table_index_q(d, c.div, c.inv_div) : table_index_d(s2, d, sT, c.inv_div)];
+        const float sc = tf + xqj.w;
+        t[0] = ux * sc * w;
+        t[1] = uy * sc * w;
+        t[2] = uz * sc * w;
+    }, acc);
+    if (quad_sub() != 0u) return;
+    const size_t ob = (size_t)S.grid[B.s].buf_of[k % LPS_MAX_ITER_SLOTS] * mk.stride;
+    const float ri = rho_pred[ob + i];
+    const float4 f4 = make_float4((-acc[0]) * c.mass * ri, (-acc[1]) * c.mass * ri, (-acc[2]) * c.mass * ri, 0.f);
+    fp[ob + i] = f4;
+    const float4 f = force[i];
+    const float4 xr = posrho[i];
+    const float4 xn = predict(make_float4(xr.x, xr.y, xr.z, 0.f), vel[i], vh[i], first[i] != 0, f.x + f4.x,
+                              f.y + f4.y, f.z + f4.z, xr.w, c, boxes0 + B.info.box0, spheres0 + B.info.sphere0,
+                              planes0 + B.info.plane0);
+    xpred_next[i] = xn;
+    mk.cq[i] = quantize(xn.x, xn.y, xn.z, G, compact_scale(G));
+    if (lps_div::tiny_coord(xn.x, xn.y, xn.z)) g->tiny_pred = 1u;
+}
+
 // integrate (SPHKernels.cl:424-455) + pciAddPressureForce (SPHKernels.cl:880-890)
 // + handleCollisions (SPHKernels.cl:457-486); accumulates the bounding box of
 // the new positions for the next step.
@@ -1890,12 +2384,26 @@ void launch_cell_table(hipStream_t s, uint32_t* run_end, const DevGrid* g, uint3
 
 void launch_density(hipStream_t s, const PhysArgs& a, float* press, float* q) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    if (a.quad) {
+        const uint32_t nq = nl * LPS_QUAD;
+        k_density_q<<<xcd_grid(nq), LPS_BLOCK, 0, s>>>(a.pos, a.g, a.sc, a.cell_lb, a.c, a.tables, a.posrho, press, q,
+                                                      a.mk, nq);
+        return;
+    }
     k_density<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.pos, a.g, a.sc, a.cell_lb, a.c, a.tables, a.posrho, press, q, a.mk,
                                                 nl);
 }
 
 void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float* pprev, float* feat) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    if (a.quad) {
+        const uint32_t nq = nl * LPS_QUAD;
+        auto kern = feat ? k_forces_q<true> : k_forces_q<false>;
+        kern<<<xcd_grid(nq), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables,
+                                                a.boxes, a.spheres, a.planes, a.force, xpred, pprev, feat, a.sc, a.mk,
+                                                nq);
+        return;
+    }
     auto kern = feat ? k_forces<true> : k_forces<false>;
     kern<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, a.vel, a.vh, a.first, a.g, a.cell_lb, a.c, a.tables, a.boxes,
                                             a.spheres, a.planes, a.force, xpred, pprev, feat, a.sc, a.mk, nl);
@@ -1903,6 +2411,12 @@ void launch_forces(hipStream_t s, const PhysArgs& a, float4* xpred, const float*
 
 void launch_pressure_sph(hipStream_t s, const PhysArgs& a, const float* q) {
     const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+    if (a.quad) {
+        const uint32_t nq = nl * LPS_QUAD;
+        k_pressure_sph_q<<<xcd_grid(nq), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, a.sc,
+                                                           a.mk, nq);
+        return;
+    }
     k_pressure_sph<<<xcd_grid(nl), LPS_BLOCK, 0, s>>>(a.posrho, q, a.g, a.cell_lb, a.c, a.tables, a.force, a.sc, a.mk,
                                                      nl);
 }
@@ -1918,6 +2432,13 @@ static void pci_density_wg(hipStream_t s, const PhysArgs& a, const float4* xpred
 
 void launch_pci_density(hipStream_t s, const PhysArgs& a, const float4* xpred, float* rho_pred, float* press,
                         float4* xq, uint32_t k, float* feat, uint32_t threads) {
+    if (a.quad) {
+        const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+        auto kern = (feat && k == 1u) ? k_pci_density_q<true> : k_pci_density_q<false>;
+        kern<<<xcd_grid(nl * LPS_QUAD), LPS_BLOCK, 0, s>>>(a.pos, xpred, a.g, a.cell_lb, a.c, a.tables, rho_pred, press,
+                                                          xq, k, feat, a.posrho, a.sc, a.mk, nl);
+        return;
+    }
     if (threads == 64u) pci_density_wg<64>(s, a, xpred, rho_pred, press, xq, k, feat);
     else if (threads == 128u) pci_density_wg<128>(s, a, xpred, rho_pred, press, xq, k, feat);
     else pci_density_wg<LPS_BLOCK>(s, a, xpred, rho_pred, press, xq, k, feat);
@@ -1935,6 +2456,13 @@ static void pci_pressure_wg(hipStream_t s, const PhysArgs& a, const float4* xq, 
 void launch_pci_pressure(hipStream_t s, const PhysArgs& a, const float4* xq, const float* rho_pred, float4* fp,
                          float4* xpred_next, uint32_t k, const LoopRule& rule, uint32_t* host_flag,
                          uint32_t threads) {
+    if (a.quad) {
+        const uint32_t nl = cdiv(a.c.n, LPS_BLOCK);
+        k_pci_pressure_q<<<xcd_grid(nl * LPS_QUAD), LPS_BLOCK, 0, s>>>(
+            a.pos, xq, rho_pred, a.g, a.cell_lb, a.c, a.tables, fp, a.force, a.posrho, a.vel, a.vh, a.first, a.boxes,
+            a.spheres, a.planes, xpred_next, k, a.sc, a.mk, rule, host_flag, nl);
+        return;
+    }
     if (threads == 64u) pci_pressure_wg<64>(s, a, xq, rho_pred, fp, xpred_next, k, rule, host_flag);
     else if (threads == 128u) pci_pressure_wg<128>(s, a, xq, rho_pred, fp, xpred_next, k, rule, host_flag);
     else pci_pressure_wg<LPS_BLOCK>(s, a, xq, rho_pred, fp, xpred_next, k, rule, host_flag);

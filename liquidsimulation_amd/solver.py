@@ -292,16 +292,17 @@ class SPHSolver:
         return d.value
 
     # ---- launch tuning
-    def set_tuning(self, use_graphs: bool, lookahead: int = 1, correction_threads: int = 256):
+    def set_tuning(self, use_graphs: bool, lookahead: int = 1, correction_threads: int = 256,
+                   lanes_per_particle: int = 0):
         t = CTuning(use_graphs=int(bool(use_graphs)), lookahead=int(lookahead), ms_per_step=0.0,
-                    correction_threads=int(correction_threads))
+                    correction_threads=int(correction_threads), lanes_per_particle=int(lanes_per_particle))
         check(lib().lps_set_tuning(self._ctx, C.byref(t)), "lps_set_tuning")
 
     def get_tuning(self):
         t = CTuning()
         check(lib().lps_get_tuning(self._ctx, C.byref(t)), "lps_get_tuning")
         return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead,
-                "correction_threads": t.correction_threads}
+                "correction_threads": t.correction_threads, "lanes_per_particle": t.lanes_per_particle}
 
     def autotune(self, dt: float, steps: int = 10):
         """Time every launch setting for `steps` steps from the current state,
@@ -310,7 +311,7 @@ class SPHSolver:
         t = CTuning()
         check(lib().lps_autotune(self._ctx, float(dt), int(steps), C.byref(t)), "lps_autotune")
         return {"use_graphs": bool(t.use_graphs), "lookahead": t.lookahead, "ms_per_step": t.ms_per_step,
-                "correction_threads": t.correction_threads}
+                "correction_threads": t.correction_threads, "lanes_per_particle": t.lanes_per_particle}
 
     # ---- timing
     def profile(self, on: bool = True):
