@@ -1282,9 +1282,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
 
 // accumulateNonPressureForces (SPHKernels.cl:256-349); for PCISPH also the
 // first prediction of the correction loop (pciInit + pciIntegrate +
-// pciHandleCollisions with fp = 0, SPHKernels.cl:661-735).
+// pciHandleCollisions with fp = 0, SPHKernels.cl:661-735).  Held one wave per
+// SIMD above what the compiler picks (7 and 6 waves: 102.3 and 110.1 µs
+// against 103.3 and 113.4 at 524k); at eight it spills inside the plane loop.
 template <bool FEAT>
-__global__ __launch_bounds__(LPS_BLOCK) void k_forces(
+__global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(FEAT ? 6 : 7))) void k_forces(
     const float4* __restrict__ posrho, const float4* __restrict__ vel, const float4* __restrict__ vh,
     const uint8_t* __restrict__ first, DevGrid* g, const uint32_t* __restrict__ cell_lb, SimConst c0,
     const float* __restrict__ tables, const lps_box* __restrict__ boxes0, const lps_sphere* __restrict__ spheres0,
