@@ -191,17 +191,19 @@ int lps_get_scene_stats(lps_ctx* ctx, lps_step_stats* stats /* [nscenes] */, uin
  * a wave that share one particle's neighbour walk in the stencil kernels (1,
  * or 4: a quad splits the candidates and adds the terms in the one-lane
  * order; 0 chooses 4 below 512 particle blocks of 256 slots, else 1; the
- * correction workgroup size then applies only with 1).  lps_autotune times
- * `steps` steps of every combination from the current state (4 lanes only
- * below 2048 particle blocks), restores the state (the run continues
- * exactly as without tuning) and keeps the fastest; every setting gives
- * bit-identical results. */
+ * correction workgroup size then applies only with 1; with several scenes, 0
+ * also gives quads to the corrections that run fewer than 512 blocks).
+ * lps_autotune times `steps` steps of every combination from the current
+ * state (4 lanes only below 2048 particle blocks, 0 only with several scenes
+ * and at least 512 blocks), restores the state (the run continues exactly as
+ * without tuning) and keeps the fastest; every setting gives bit-identical
+ * results. */
 typedef struct lps_tuning {
     int32_t use_graphs;            /* 0 or 1 */
     uint32_t lookahead;            /* 1..4 */
     float ms_per_step;             /* lps_autotune: measured time of the chosen setting */
     uint32_t correction_threads;   /* 64, 128 or 256 */
-    uint32_t lanes_per_particle;   /* 0 (automatic), 1 or 4; lps_get_tuning reports the one in use */
+    uint32_t lanes_per_particle;   /* 0 (automatic), 1 or 4; lps_get_tuning reports the step's static choice */
 } lps_tuning;
 int lps_set_tuning(lps_ctx* ctx, const lps_tuning* tuning);
 int lps_get_tuning(lps_ctx* ctx, lps_tuning* tuning);

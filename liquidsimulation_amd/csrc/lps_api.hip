@@ -168,7 +168,7 @@ struct lps_ctx {
     uint32_t* d_actlist = nullptr;         // [2][blocks] active-block lists of the correction kernels
 
     // host
-    uint32_t* flags = nullptr;  // pinned, coherent: 4 words per iteration slot
+    uint32_t* flags = nullptr;  // pinned, coherent: LPS_FLAG_WORDS words per iteration slot
     DevGrid* h_grid = nullptr;  // pinned copy
     // coherent: per scene kinetic and potential energy of the state entering
     // the last step (written by its grid set-up) or of the current state
@@ -630,7 +630,9 @@ LoopRule loop_rule(const lps_ctx* c) {
 // converged}, written by the device (word 0 last).  The host marks it
 // pending before the correction is enqueued and polls word 0.
 constexpr uint32_t FLAG_PENDING = 0xFFFFFFFFu;
-volatile uint32_t* flag_slot(lps_ctx* c, uint32_t kk) { return c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS); }
+volatile uint32_t* flag_slot(lps_ctx* c, uint32_t kk) {
+    return c->flags + LPS_FLAG_WORDS * (kk % LPS_MAX_ITER_SLOTS);
+}
 
 // Waits until the device has written slot kk; false after ~10 s without it.
 bool wait_flag(lps_ctx* c, uint32_t kk) {
@@ -670,7 +672,7 @@ void enqueue_iter(lps_ctx* c, const lps_dev::PhysArgs& a, uint32_t kk, const Loo
     {
         Timed t(c, "pci_pressure");
         lps_dev::launch_pci_pressure(s, a, c->xq, c->rho_pred, c->fp, c->xpred[kk & 1u], kk, rule,
-                                     c->flags + 4 * (kk % LPS_MAX_ITER_SLOTS), c->corr_threads);
+                                     c->flags + LPS_FLAG_WORDS * (kk % LPS_MAX_ITER_SLOTS), c->corr_threads);
     }
 }
 
@@ -763,11 +765,14 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
     lps_dev::PhysArgs a = phys_args(c, k);
     // Corrections beyond min: `lookahead` corrections stay queued ahead of
     // the result being waited for; queued corrections past the stop return
-    // at once on the device.
+    // at once on the device.  With automatic lanes, a correction enqueued
+    // after the host learned that fewer than 512 particle blocks still run
+    // (the late corrections of an ensemble) gives each particle a quad.
     uint32_t k_enq = kmin, k_known = 0;
+    lps_dev::PhysArgs ac = a;
     while (true) {
         const uint32_t kw = std::max(k_known + 1u, kmin);
-        while (k_enq < kmax && k_enq < kw + c->lookahead) enqueue_iter(c, a, ++k_enq, rule, feat, seeded, delta, true);
+        while (k_enq < kmax && k_enq < kw + c->lookahead) enqueue_iter(c, ac, ++k_enq, rule, feat, seeded, delta, true);
         if (!wait_flag(c, kw)) {
             res.ok = false;  // the device never answered: reported as a HIP error
             res.iterations = -1;
@@ -782,6 +787,7 @@ StepResult enqueue_step(lps_ctx* c, float dt, bool seeded, bool feat_on) {
         if (state != 0u) break;
         k_known = kw;
         if (kw >= kmax) break;
+        if (c->lanes == 0u && f[4] < 512u) ac.quad = 1u;
     }
     {
         Timed t(c, "integrate");
@@ -925,7 +931,7 @@ int lps_create(const lps_params* params, lps_ctx** out) {
         return LPS_ERR_HIP;
     }
     bool ok = hipMalloc((void**)&c->grid, sizeof(DevGrid)) == hipSuccess &&
-              hipHostMalloc((void**)&c->flags, sizeof(uint32_t) * 4 * LPS_MAX_ITER_SLOTS,
+              hipHostMalloc((void**)&c->flags, sizeof(uint32_t) * LPS_FLAG_WORDS * LPS_MAX_ITER_SLOTS,
                             hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
               hipHostMalloc((void**)&c->h_grid, sizeof(DevGrid), hipHostMallocDefault) == hipSuccess &&
               hipHostMalloc((void**)&c->h_energy, sizeof(double) * 2 * LPS_MAX_SCENES,
@@ -941,7 +947,7 @@ int lps_create(const lps_params* params, lps_ctx** out) {
         lps_destroy(c);
         return LPS_ERR_OOM;
     }
-    std::memset(c->flags, 0, sizeof(uint32_t) * 4 * LPS_MAX_ITER_SLOTS);
+    std::memset(c->flags, 0, sizeof(uint32_t) * LPS_FLAG_WORDS * LPS_MAX_ITER_SLOTS);
     *out = c;
     return LPS_OK;
 }
@@ -1503,14 +1509,23 @@ int lps_autotune(lps_ctx* c, float dt, uint32_t steps, lps_tuning* chosen) {
     // graphs x lookahead x correction-kernel workgroup size (the particle
     // block of 256 slots run as 1, 2 or 4 workgroups) with one lane per
     // particle; below 2048 particle blocks also graphs x lookahead with a
-    // quad per particle (where the workgroup size does not apply)
+    // quad per particle (where the workgroup size does not apply); with
+    // several scenes and at least 512 blocks also the automatic lanes (one
+    // lane, and quads in the corrections that run fewer than 512 blocks)
     struct Cand { bool g; uint32_t l, t, q; float ms; };
+    std::vector<uint32_t> lane_opts = {1u};
+    if (n / LPS_BLOCK < 2048u) lane_opts.push_back(4u);
+    if (c->h_sinfo.size() > 1 && n / LPS_BLOCK >= 512u) lane_opts.push_back(0u);
     std::vector<Cand> cands;
     for (int g = 0; g < 2; g++)
-        for (uint32_t l = 1; l <= 3; l++) {
-            for (uint32_t t : {(uint32_t)LPS_BLOCK, 128u, 64u}) cands.push_back({g == 1, l, t, 1u, 0.f});
-            if (n / LPS_BLOCK < 2048u) cands.push_back({g == 1, l, t0, 4u, 0.f});
-        }
+        for (uint32_t l = 1; l <= 3; l++)
+            for (uint32_t q : lane_opts) {
+                if (q == 4u) {
+                    cands.push_back({g == 1, l, t0, 4u, 0.f});
+                    continue;
+                }
+                for (uint32_t t : {(uint32_t)LPS_BLOCK, 128u, 64u}) cands.push_back({g == 1, l, t, q, 0.f});
+            }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!st && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)) st = LPS_ERR_HIP;
     for (Cand& cd : cands) {

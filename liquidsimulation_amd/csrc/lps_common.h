@@ -16,6 +16,9 @@
 #define LPS_SCAN_ITEMS 4       // cells per thread in the cell-table scan
 #define LPS_SCAN_TILE (LPS_BLOCK * LPS_SCAN_ITEMS)
 #define LPS_MAX_ITER_SLOTS 64   // per-correction error slots, used as a ring: correction k owns slot (k - 1) % 64
+// host flag words per correction slot: state, corrections, error bits,
+// converged, particle blocks the next correction runs (then unused)
+#define LPS_FLAG_WORDS 8
 #define LPS_MAX_COLLIDERS 1024    // per context (all scenes)
 #define LPS_FEAT 16              // warm-start features per particle (LPS_MLP_IN)
 

@@ -1886,8 +1886,9 @@ __global__ __launch_bounds__(WG) void k_pci_density(
 // below the tolerance (converged), when k >= min and the maximum error has not
 // improved on its best value for `stall_limit` corrections, or at k == max; a
 // NaN error marks the step non-finite.  Builds the next correction's block
-// list and writes {state, corrections, error bits, converged} to the host slot
-// (state 0 continue, 1 stopped, 2 error).  The rest of the pressure pass reads
+// list and writes {state, corrections, error bits, converged, particle blocks
+// of the next correction} to the host slot (state 0 continue, 1 stopped, 2
+// error).  The rest of the pressure pass reads
 // none of what it writes except stop_iter (a scene stopping at k has
 // stop_iter = k, which the pass treats like 0) and the step status.  A
 // correction queued past the stop answers nothing: the host never waits on
@@ -2018,6 +2019,7 @@ __device__ void loop_rule_block(DevGrid* g, const lps_dev::Scenes& S, uint32_t k
     host_flag[1] = k;
     host_flag[2] = s_err;
     host_flag[3] = s_conv;
+    host_flag[4] = g->act_count[next];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");  // as above: words 1-3 before word 0
     host_flag[0] = state;
 }

@@ -175,3 +175,32 @@ def test_ensemble_64_workload(gpu, oracle, warm):
         print(f"scene {k}: n={m.n} oracle corrections {tr.iterations}", flush=True)
         for key, ref in (("x", st.x), ("v", st.v), ("vhalf", st.vh)):
             assert ulp_diff(pe[key][sl], ref) == 0, (k, key)
+
+
+def test_late_corrections_in_quads_match_one_lane(gpu):
+    """Four scenes of 40,176 particles (628 particle blocks in all): with
+    automatic lanes the corrections enqueued after fewer than 512 blocks
+    still run (a scene stopped) give each particle a quad; the states and the
+    per-scene corrections equal those of one lane per particle throughout."""
+    mem = _members(METHOD_PCISPH, [(36, 36, 31)] * 4)
+    base = mem[0][0].params
+    runs = []
+    for lanes in (1, 0):
+        e = Ensemble(base)
+        e.set_scenes([m[0].x for m in mem], [m[0].v for m in mem], gravity=[m[2] for m in mem])
+        for k, (_, col, _) in enumerate(mem):
+            e.set_scene_colliders(k, col)
+        e.set_tuning(False, 1, 256, lanes)
+        its = []
+        for _ in range(6):
+            e.update(mem[0][0].dt, 1)
+            its.append([st.iterations for st in e.scene_stats()])
+        runs.append((e.get_particles(), its))
+        e.close()
+    (p1, its1), (p0, its0) = runs
+    print("per-scene corrections:", its1, flush=True)
+    assert its0 == its1
+    # some step ran on after a scene had stopped (the quads' case)
+    assert any(len(set(row)) > 1 for row in its1)
+    for key in ("x", "v", "vhalf"):
+        assert ulp_diff(p0[key], p1[key]) == 0, key
