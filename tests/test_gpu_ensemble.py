@@ -204,3 +204,32 @@ def test_late_corrections_in_quads_match_one_lane(gpu):
     assert any(len(set(row)) > 1 for row in its1)
     for key in ("x", "v", "vhalf"):
         assert ulp_diff(p0[key], p1[key]) == 0, key
+
+
+def test_autotune_of_an_ensemble_leaves_the_trajectory_unchanged(gpu):
+    """The autotuner on a context of several scenes and 628 particle blocks
+    (it tries one lane, quads and the automatic lanes there) restores the
+    state: the run continues exactly as without tuning."""
+    mem = _members(METHOD_PCISPH, [(36, 36, 31)] * 4)
+    base = mem[0][0].params
+    runs = []
+    for tune in (False, True):
+        e = Ensemble(base)
+        e.set_scenes([m[0].x for m in mem], [m[0].v for m in mem], gravity=[m[2] for m in mem])
+        for k, (_, col, _) in enumerate(mem):
+            e.set_scene_colliders(k, col)
+        e.update(mem[0][0].dt, 2)
+        chosen = e.autotune(mem[0][0].dt, 2) if tune else None
+        if tune:
+            print("autotune chose", chosen, flush=True)
+            assert chosen["lanes_per_particle"] in (0, 1, 4)
+        its = []
+        for _ in range(4):
+            e.update(mem[0][0].dt, 1)
+            its.append([st.iterations for st in e.scene_stats()])
+        runs.append((e.get_particles(), its))
+        e.close()
+    (p0, its0), (p1, its1) = runs
+    assert its0 == its1
+    for key in ("x", "v", "vhalf"):
+        assert ulp_diff(p0[key], p1[key]) == 0, key
