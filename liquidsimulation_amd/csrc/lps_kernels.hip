@@ -1504,26 +1504,46 @@ __device__ __forceinline__ uint32_t quad_filter(const char* __restrict__ xb, con
     return quad_or(bits) & (n == 32u ? 0xFFFFFFFFu : (1u << n) - 1u);
 }
 
+// The quad's next trip from `mask` (advanced past it): this lane's candidate
+// j (w0 when it has none: a valid index to load), whether it has one, and
+// how many of the quad's lanes have one (0: the window is done).
+struct QuadTrip {
+    uint32_t j, cnt;
+    bool have;
+};
+__device__ __forceinline__ QuadTrip quad_trip(uint32_t w0, uint32_t& mask) {
+    const uint32_t sub = quad_sub();
+    uint32_t mm = mask;
+    if (sub > 0u) mm &= mm - 1u;
+    if (sub > 1u) mm &= mm - 1u;
+    if (sub > 2u) mm &= mm - 1u;
+    QuadTrip r;
+    r.cnt = min((uint32_t)__popc(mask), LPS_QUAD);
+#pragma unroll
+    for (uint32_t t = 0; t < LPS_QUAD; t++) mask &= mask - 1u;
+    r.have = mm != 0u;
+    r.j = r.have ? w0 + (uint32_t)__ffs((int)mm) - 1u : w0;
+    return r;
+}
+
 // Pass 2 of one window for a quad: `term(ok, j, x_j, s2, pre(j), x_j - x_i, t)`
 // writes the NT terms of candidate j (zeros when not ok), and acc[a] takes
-// the terms of the quad's candidates in order.
+// the terms of the quad's candidates in order.  The loads of the next trip
+// issue before this trip's terms are computed: a small scene has about one
+// wave per SIMD, which would otherwise wait out every gather.
 template <int NT, class Acc, class Pre, class Term>
 __device__ __forceinline__ void quad_visit(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px, float py,
                                            float pz, float s_max, Acc& accept_extra, Pre& pre, Term& term,
                                            float (&acc)[NT]) {
-    const uint32_t sub = quad_sub();
-    while (mask) {
-        uint32_t mm = mask;
-        if (sub > 0u) mm &= mm - 1u;
-        if (sub > 1u) mm &= mm - 1u;
-        if (sub > 2u) mm &= mm - 1u;
-        const uint32_t cnt = min((uint32_t)__popc(mask), LPS_QUAD);
-#pragma unroll
-        for (uint32_t t = 0; t < LPS_QUAD; t++) mask &= mask - 1u;
-        const bool have = mm != 0u;
-        const uint32_t j = have ? w0 + (uint32_t)__ffs((int)mm) - 1u : w0;
-        const float4 q = *reinterpret_cast<const float4*>(xb + (j << 4));
-        const auto pv = pre(j);
+    QuadTrip tr = quad_trip(w0, mask);
+    float4 q = *reinterpret_cast<const float4*>(xb + (tr.j << 4));
+    auto pv = pre(tr.j);
+    while (tr.cnt) {
+        const QuadTrip nx = quad_trip(w0, mask);
+        const float4 qn = *reinterpret_cast<const float4*>(xb + (nx.j << 4));
+        const auto pvn = pre(nx.j);
+        const uint32_t j = tr.j, cnt = tr.cnt;
+        const bool have = tr.have;
         const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
         const float s2 = dx * dx + dy * dy + dz * dz;
         float t[NT];
@@ -1542,6 +1562,9 @@ __device__ __forceinline__ void quad_visit(const char* __restrict__ xb, uint32_t
 #pragma unroll
             for (int a = 0; a < NT; a++) acc[a] += quad_bcast<3>(t[a]);
         }
+        tr = nx;
+        q = qn;
+        pv = pvn;
     }
 }
 
