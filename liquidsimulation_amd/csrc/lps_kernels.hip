@@ -250,14 +250,6 @@ __device__ __forceinline__ uint32_t compact_diff(const uint2& e, const uint2& qi
     return (uint32_t)__builtin_amdgcn_sdot2(dxy, dxy, __builtin_amdgcn_sdot2(dz, dz, bias, true), true);
 }
 
-// Pass 2 visits VB kept candidates per trip (their loads issue together);
-// the float filter tests four candidates per trip.  VB = 2 keeps every
-// stencil kernel at or below 64 VGPRs, i.e. eight waves per SIMD: the rare
-// paths (the float filter of a far particle or of a replayed row's later
-// windows, and the checked division) otherwise set the register count of the
-// whole kernel.  A row holds about four kept candidates, so VB = 3 or 4
-// mostly fills slots with repeats, and VB = 1 leaves one load in flight;
-// both measured slower at 524k particles.
 // A visitor called for every slot of a trip with whether the slot is an
 // accepted pair, for branch-free bodies.  The density sums add +0 for the
 // other slots, which leaves a sum of non-negative terms (never -0) unchanged,
@@ -284,6 +276,14 @@ __device__ __forceinline__ void visit_slot(F& fn, bool ok, uint32_t j, const flo
         fn(j, q, s2, pv, dl);
 }
 
+// Pass 2 visits VB kept candidates per trip (their loads issue together);
+// the float filter tests four candidates per trip.  With VB = 2 the recording
+// passes stay at or below 64 VGPRs, i.e. eight waves per SIMD: the rare
+// paths (the float filter of a far particle or of a replayed row's later
+// windows, and the checked division) otherwise set the register count of the
+// whole kernel.  A row holds about four kept candidates, so VB = 3 or 4
+// mostly fills slots with repeats, and VB = 1 leaves one load in flight; in
+// the recording passes all measure within 1 % at 524k particles.
 template <int VB, class Acc, class Pre, class F>
 __device__ __forceinline__ void visit_window(const char* __restrict__ xb, uint32_t w0, uint32_t mask, float px,
                                              float py, float pz, float s_max, Acc& accept_extra, Pre& pre, F& fn) {
