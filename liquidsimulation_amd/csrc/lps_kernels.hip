@@ -1,14 +1,15 @@
 // lps_kernels.hip -- hand-written gfx950 kernels of the SPH/PCISPH step.
 //
 // One launch per phase of PhysicSolver::update (LiquidPhysics/src/PhysicSolver.cpp:12-21):
-//   grid      k_bbox, k_grid_setup, k_keys, k_radix_{hist,scan,scatter} x P,
-//             k_reorder (+ run-end marks), k_cell_{reduce,scan}
+//   grid      k_grid_one (or k_scene_grid + k_grid_finish), k_keys,
+//             k_radix_{hist,scan,scatter} x P, k_reorder (+ run-end marks), k_cell_scan
 //   density   k_density
 //   forces    k_forces (+ first PCISPH prediction)
-//   pressure  SPH: k_pressure_sph;  PCISPH: {k_pci_density, k_pci_pressure (+ next
-//             prediction), k_pci_check} per correction
+//   pressure  SPH: k_pressure_sph;  PCISPH: {k_pci_density, k_pci_pressure (+ the
+//             loop rule in block 0, + next prediction)} per correction
 //   integrate k_integrate (+ collisions, + bounding box of the next step)
-// Particle kernels run 256 threads (4 wave64) per block, one particle per thread,
+// Particle kernels run 256 threads (4 wave64) per block, one particle per thread
+// (the *_q variants of the stencil kernels: a quad of threads per particle),
 // with a block index remapped so each of the 8 XCDs works on a contiguous range
 // of the cell-sorted particles and keeps its neighbours in its own L2.
 #include <hip/hip_runtime.h>
