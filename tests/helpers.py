@@ -9,8 +9,12 @@ from liquidsimulation_amd.solver import solver_for
 from oracle import pyoracle as O
 
 
-def gpu_run(params: Params, col: Colliders, dt: float, st: "O.State", nsteps: int = 1):
+def gpu_run(params: Params, col: Colliders, dt: float, st: "O.State", nsteps: int = 1, lanes=None):
+    """The library's run from `st`; lanes (1 or 4) fixes the lanes per
+    particle of the stencil kernels, None leaves the automatic choice."""
     s = solver_for(params)
+    if lanes is not None:
+        s.set_tuning(True, 1, 256, lanes)
     s.set_colliders(col)
     s.set_particles(st.x, st.v, st.vh, st.first)
     stats = s.update(dt, nsteps)

@@ -26,11 +26,11 @@ from tests.helpers import gpu_run, rel_err, ulp_diff
 pytestmark = pytest.mark.gpu
 
 
-def _compare_step(oracle, sc, st, nsteps_before=0):
+def _compare_step(oracle, sc, st, nsteps_before=0, lanes=None):
     O = oracle
     a = st.copy()
     tr = O.step_ocl(sc.params, sc.colliders, sc.dt, a)
-    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 1)
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 1, lanes=lanes)
     g = s.get_grid()
     f = s.get_fields()
     p = s.get_particles()
@@ -120,12 +120,14 @@ def test_multi_step_state_matches(gpu, oracle, method):
     assert ulp_diff(p["v"], a.v) == 0
 
 
-def test_tiny_coordinates_take_the_checked_division(gpu, oracle):
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_tiny_coordinates_take_the_checked_division(gpu, oracle, lanes):
     """Predictions with a coordinate in (0, 2^-56) switch the pressure pass to
     the quotient guard that checks every numerator.  The scene is shifted so
     that z = 0 lies mid-box; two particles far from the block sit at z = 1e-30
     and 3e-30 (no z velocity, z forces of order 1e-28), so their predictions
-    keep a tiny z and their mutual difference is tiny too (IEEE division)."""
+    keep a tiny z and their mutual difference is tiny too (IEEE division).
+    With one lane and with a quad of lanes per particle."""
     import dataclasses
     sc = scenes.small_block(10, 12, 9, method=METHOD_PCISPH)
     b = sc.colliders.boxes[0]
@@ -135,20 +137,22 @@ def test_tiny_coordinates_take_the_checked_division(gpu, oracle):
     extra = np.array([[0.7, 0.3, 1e-30], [0.7, 0.3 + 0.5 * sc.params.particle_radius, 3e-30]], np.float32)
     sc.x = np.concatenate([x, extra]).astype(np.float32)
     st = oracle.State.from_scene(sc.x)
-    s, _, a = _compare_step(oracle, sc, st)
+    s, _, a = _compare_step(oracle, sc, st, lanes=lanes)
     z = np.abs(a.x[:, 2])
     assert 0.0 < float(np.min(z[z > 0])) < 2.0 ** -56  # the predictions kept a tiny coordinate
     s.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
 @pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
-def test_far_cluster_takes_the_float_filter(gpu, oracle, method):
+def test_far_cluster_takes_the_float_filter(gpu, oracle, method, lanes):
     """Two clusters 700 cells apart along x (no colliders).  The stencil's
     first pass tests candidates on 16-bit fixed-point copies of the positions
     (4096 units per cell) only while a particle lies below 2^21 units (512
     cells) from the grid origin; the far cluster lies beyond and filters with
     floats, in the density pass and, for PCISPH, the predicted-density pass.
-    Every field stays bit-exact."""
+    Every field stays bit-exact, with one lane and with a quad of lanes per
+    particle."""
     sc = scenes.small_block(8, 8, 8, method=method)
     sc.colliders.boxes = []
     h = sc.params.particle_radius * sc.params.kernel_radius_factor
@@ -158,7 +162,7 @@ def test_far_cluster_takes_the_float_filter(gpu, oracle, method):
     span_cells = (float(sc.x[:, 0].max()) - float(sc.x[:, 0].min())) / h
     assert span_cells * 4096.0 > 2.0 ** 21 * 1.2  # the far cluster is past the fixed-point range
     st = oracle.State.from_scene(sc.x)
-    s, _, _ = _compare_step(oracle, sc, st)
+    s, _, _ = _compare_step(oracle, sc, st, lanes=lanes)
     s.close()
 
 
