@@ -102,9 +102,11 @@ def test_sph_16k_single_step(gpu, oracle):
     _compare_step(oracle, sc, st)
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
 @pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
-def test_multi_step_state_matches(gpu, oracle, method):
-    """20 steps on each side from the same start; the trajectories stay bit-identical."""
+def test_multi_step_state_matches(gpu, oracle, method, lanes):
+    """20 steps on each side from the same start; the trajectories stay
+    bit-identical, with one lane and with a quad of lanes per particle."""
     sc = scenes.small_block(12, 16, 10, method=method, dt=0.002)
     st = oracle.State.from_scene(sc.x)
     a = st.copy()
@@ -112,7 +114,7 @@ def test_multi_step_state_matches(gpu, oracle, method):
     for _ in range(20):
         tr = oracle.step_ocl(sc.params, sc.colliders, sc.dt, a)
         its.append(tr.iterations)
-    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 20)
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 20, lanes=lanes)
     p = s.get_particles()
     print("iterations gpu", [x.iterations for x in stats], "oracle", its, flush=True)
     assert [x.iterations for x in stats] == its
@@ -271,9 +273,10 @@ VARIANTS = {
 }
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
 @pytest.mark.parametrize("method", [METHOD_SPH, METHOD_PCISPH])
-def test_parameter_variants_multi_step(gpu, oracle, method, variant):
+def test_parameter_variants_multi_step(gpu, oracle, method, variant, lanes):
     from liquidsimulation_amd.abi import Box
     sc = scenes.small_block(9, 11, 7, method=method, dt=0.002, **VARIANTS[variant])
     obstacle = variant.startswith("inelastic")
@@ -302,7 +305,7 @@ def test_parameter_variants_multi_step(gpu, oracle, method, variant):
         for _ in range(12):
             oracle.step_ocl(sc.params, col, sc.dt, free)
         assert not np.array_equal(free.x, a.x)
-    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 12)
+    s, stats = gpu_run(sc.params, sc.colliders, sc.dt, st, 12, lanes=lanes)
     p = s.get_particles()
     print(f"[{variant}] iterations gpu", [x.iterations for x in stats], "oracle", its, flush=True)
     assert [x.iterations for x in stats] == its
