@@ -129,7 +129,11 @@ int lps_index_thresholds(const lps_params* params, float* thr /* [LPS_TABLE_SIZE
 
 /* Scene.  lps_set_particles replaces the upload of SPHSolver::initParallelBuffers
  * (SPHSolver.cpp:788-886); arrays are xyz-interleaved [3n]; vhalf and first may
- * be NULL (zero half velocity, first step set).  1 <= n <= LPS_MAX_PARTICLES. */
+ * be NULL (zero half velocity, first step set).  n >= 1, and n rounded up to
+ * whole particle blocks of 256 slots is at most LPS_MAX_PARTICLES (so
+ * n <= 2^28 - 256); other counts are refused with LPS_ERR_INVALID before any
+ * array is read.  The same bound holds for the slots of all scenes together
+ * (lps_set_scenes) and after lps_add_particles. */
 #define LPS_MAX_PARTICLES ((1u << 28) - 1u)
 int lps_set_particles(lps_ctx* ctx, const float* x, const float* v, const float* vhalf,
                       const uint8_t* first, uint32_t n);

@@ -1137,7 +1137,8 @@ int lps_set_scenes(lps_ctx* c, uint32_t nscenes, const uint32_t* counts, const f
 
 int lps_set_particles(lps_ctx* c, const float* x, const float* v, const float* vhalf, const uint8_t* first,
                       uint32_t n) {
-    if (!c || !x || n == 0 || n > LPS_MAX_PARTICLES - LPS_BLOCK) return LPS_ERR_INVALID;
+    if (!c || !x || n == 0) return LPS_ERR_INVALID;
+    // the slot bound is lps_set_scenes's
     return lps_set_scenes(c, 1u, &n, nullptr, x, v, vhalf, first);
 }
 
@@ -1145,13 +1146,13 @@ int lps_add_particles(lps_ctx* c, const float* x, const float* v, const float* v
                       uint32_t n) {
     if (!c || !x || n == 0) return LPS_ERR_INVALID;
     if (c->h_sinfo.size() != 1u || c->n_real == 0) return LPS_ERR_STATE;
-    for (uint64_t k = 0; k < 3ull * n; k++)
-        if (!std::isfinite(x[k]) || (v && !std::isfinite(v[k])) || (vhalf && !std::isfinite(vhalf[k])))
-            return LPS_ERR_INVALID;
     const uint32_t old = c->n_real;
     const uint64_t total = (uint64_t)old + n;
     const uint64_t slots = (total + LPS_BLOCK - 1u) / LPS_BLOCK * LPS_BLOCK;
     if (slots > LPS_MAX_PARTICLES) return LPS_ERR_INVALID;
+    for (uint64_t k = 0; k < 3ull * n; k++)
+        if (!std::isfinite(x[k]) || (v && !std::isfinite(v[k])) || (vhalf && !std::isfinite(vhalf[k])))
+            return LPS_ERR_INVALID;
     const hipStream_t s = c->stream;
     HIPC(hipStreamSynchronize(s));
     int st;

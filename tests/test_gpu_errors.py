@@ -11,6 +11,9 @@ its solver has no failure path, LiquidPhysics/src/PhysicSolver.cpp:12-21).
   or not finite: LPS_ERR_INVALID, nothing changed.  (Fewer than one minimum
   correction and a negative stall count are not errors: the loop takes them
   as 1 and as no stall rule.)
+* An empty upload, or more particles than whole 256-slot blocks below
+  LPS_MAX_PARTICLES hold (one scene, several scenes, or added mid-run):
+  LPS_ERR_INVALID before any array is read, nothing changed.
 """
 import ctypes as C
 
@@ -76,6 +79,33 @@ def test_invalid_parameters_are_refused_and_change_nothing(gpu, bad):
         s.set_params(sc.params.replace(**bad))
     assert err.value.status == LPS_ERR_INVALID
     assert s.params == sc.params
+    st = s.update(sc.dt, 1)
+    assert st[0].status == 0
+    s.close()
+
+
+def test_empty_and_oversized_uploads_are_refused(gpu):
+    """Particle counts outside the documented range are refused with
+    LPS_ERR_INVALID before any array is read (the oversized counts below are
+    passed with a small array), and the uploaded state keeps stepping."""
+    sc = scenes.small_block(6, 6, 6)
+    s = solver_for(sc.params)
+    s.set_colliders(sc.colliders)
+    s.set_particles(sc.x)
+    L = lib()
+    x = np.ascontiguousarray(sc.x, np.float32).reshape(-1)
+    too_many = (1 << 28) - 255  # 2^28 slots once padded to whole blocks of 256
+    for n in (0, too_many, 0xFFFFFFFF):
+        assert L.lps_set_particles(s._ctx, x, None, None, None, n) == LPS_ERR_INVALID, n
+    for counts in ([1 << 27, 1 << 27], [5, 0], [1] * 257):
+        c = np.asarray(counts, np.uint32)
+        assert L.lps_set_scenes(s._ctx, len(counts), c, None, x, None, None, None) == LPS_ERR_INVALID, counts[:3]
+    n0 = sc.x.shape[0]
+    for n in (0, (1 << 28) - 255 - n0):
+        assert L.lps_add_particles(s._ctx, x, None, None, None, n) == LPS_ERR_INVALID, n
+    cnt = C.c_uint32(0)
+    assert L.lps_get_count(s._ctx, C.byref(cnt)) == 0 and cnt.value == n0
+    np.testing.assert_array_equal(s.get_particles()["x"], sc.x)
     st = s.update(sc.dt, 1)
     assert st[0].status == 0
     s.close()
