@@ -119,3 +119,31 @@ def test_quad_lanes_match_one_lane(gpu, method, radius_factor):
             assert ulp_diff(out[k], ref[k]) == 0, k
         for k in ("rho", "pressure", "force"):
             assert ulp_diff(f[k], f_ref[k]) == 0, k
+
+
+@pytest.mark.parametrize("dims,auto_lanes", [((73, 56, 32), 4), ((64, 64, 32), 1)])
+def test_lane_choice_at_the_block_threshold(gpu, dims, auto_lanes):
+    """130,816 particles (511 particle blocks) run a quad per particle by
+    default and 131,072 (512 blocks) one lane; over 6 PCISPH steps the
+    automatic choice and the other setting give the same state, iteration
+    counts and fields bit for bit."""
+    from liquidsimulation_amd.abi import METHOD_PCISPH
+    sc = scenes.small_block(*dims, s=0.016, jitter_amp=2e-4, seed=13, method=METHOD_PCISPH)
+    assert sc.n == dims[0] * dims[1] * dims[2]
+    outs = []
+    for lanes in (0, 5 - auto_lanes):
+        s = PCISPHSolver(sc.params)
+        s.set_colliders(sc.colliders)
+        s.set_particles(sc.x, sc.v)
+        s.set_tuning(False, 1, 256, lanes)
+        assert s.get_tuning()["lanes_per_particle"] == (auto_lanes if lanes == 0 else lanes)
+        its = [st.iterations for st in s.update(sc.dt, 6)]
+        outs.append((s.get_particles(), its, s.get_fields()))
+        s.close()
+    (ref, its_ref, f_ref), (out, its, f) = outs
+    print("iterations:", its_ref, flush=True)
+    assert its == its_ref
+    for k in ("x", "v", "vhalf"):
+        assert ulp_diff(out[k], ref[k]) == 0, k
+    for k in ("rho", "pressure", "force"):
+        assert ulp_diff(f[k], f_ref[k]) == 0, k
