@@ -119,17 +119,24 @@ __device__ __forceinline__ BlockScene block_scene(const lps_dev::Scenes& S, uint
 // list L are split XCD-contiguously over the 8 XCDs, so the pieces of one
 // block share an L2.  Returns the particle block and, in first_slot, the
 // workgroup's first slot; UINT_MAX when the workgroup has no piece.
+//
+// A single scene's lists hold all of its blocks in order while its loop runs
+// (a list is rebuilt only when a scene stops, and the blocks of the
+// corrections queued past its stop return at the stop check instead), so
+// there the piece follows from the block index alone and the workgroup
+// starts its particles' loads without two dependent reads.
 template <int WG>
 __device__ __forceinline__ uint32_t active_piece(const DevGrid* g, const lps_dev::Scenes& S, uint32_t L,
                                                  uint32_t& first_slot) {
     constexpr uint32_t P = LPS_BLOCK / WG;
-    const uint32_t act = g->act_count[L] * P;
+    const bool one = S.count == 1u;
+    const uint32_t act = (one ? S.nblocks : g->act_count[L]) * P;
     const uint32_t per = (act + LPS_XCDS - 1) / LPS_XCDS;
     const uint32_t b = blockIdx.x;
     if (b / LPS_XCDS >= per) return UINT_MAX;
     const uint32_t lp = (b % LPS_XCDS) * per + b / LPS_XCDS;
     if (lp >= act) return UINT_MAX;
-    const uint32_t lb = S.act_list[L * S.nblocks + lp / P];
+    const uint32_t lb = one ? lp / P : S.act_list[L * S.nblocks + lp / P];
     first_slot = lb * LPS_BLOCK + (lp % P) * (uint32_t)WG;
     return lb;
 }
