@@ -1847,21 +1847,24 @@ __global__ __launch_bounds__(WG) void k_pci_density(
     if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[WG / 64];
-    const BlockScene B = block_scene(S, lb);
     const bool active = true;
+    const uint32_t i = active ? s0 + threadIdx.x : UINT_MAX;
+    // the slot's own state loads while the tables are staged (every slot of
+    // an active piece lies within the allocated slots)
+    const float4 xa = pos[i];
+    const float4 xi = xpred[i];
+    const uint2 qi = mk.cq[i];
+    const BlockScene B = block_scene(S, lb);
     load_table(sW, tables);
     load_thresholds(sT, tables);
     __syncthreads();
-    const uint32_t i = active ? s0 + threadIdx.x : UINT_MAX;
     uint32_t ebits = 0;
     if (active && i < B.info.slot0 + B.info.count) {
         const GridView G = grid_view(g, S.grid + B.s);
-        const float4 xa = pos[i];
-        const float4 xi = xpred[i];
         float sum = 0.f;
         const float qs = compact_scale(G);
         CompactFilter cf{};
-        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
+        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, qi, compact_bias(c.s_max, qs)};
         // recorded for the pressure pass of this correction (same positions and cells)
         for_neighbors_masked<2>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                              MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
@@ -2075,6 +2078,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
     uint32_t s0 = 0u;
     const uint32_t lb = active_piece<WG>(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS], s0);
     if (lb == UINT_MAX) return;
+    const uint32_t i = s0 + threadIdx.x;
+    // the slot's own state loads while the tables are staged (every slot of
+    // an active piece lies within the allocated slots); the predicted-density
+    // pass's (prediction, p / rho*^2)
+    const float4 xa = pos[i];
+    const float4 xi = xq[i];
     const BlockScene B = block_scene(S, lb);
     const uint32_t stop = S.grid[B.s].stop_iter;
     if (stop != 0u && stop < k) return;
@@ -2082,16 +2091,12 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
     load_thresholds(sT, tables);
     __syncthreads();
-    const uint32_t i = s0 + threadIdx.x;
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
     const SimConst c = scene_const(c0, B.info);
     const lps_box* boxes = boxes0 + B.info.box0;
     const lps_sphere* spheres = spheres0 + B.info.sphere0;
     const lps_plane* planes = planes0 + B.info.plane0;
-    const float4 xa = pos[i];
-    // the predicted-density pass's (prediction, p / rho*^2)
-    const float4 xi = xq[i];
     const float tf = xi.w;
     // accepted pairs have d < 2^40; the predictions' coordinates were checked
     // when they were written; the common case also takes the table index as
