@@ -2087,9 +2087,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
     const BlockScene B = block_scene(S, lb);
     const uint32_t stop = S.grid[B.s].stop_iter;
     if (stop != 0u && stop < k) return;
+    // accepted pairs have d < 2^40; the predictions' coordinates were checked
+    // when they were written; the common case also takes the table index as
+    // the quotient d / div and needs no thresholds (one instantiation of the
+    // stencil per case, so the pair loop carries no mode test)
+    const bool fastdiv = c0.s_max < 0x1p78f;
+    const bool quick = fastdiv && g->tiny_pred == 0u && index_q_ok(c0.div);
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
-    load_thresholds(sT, tables);
+    if (!quick) load_thresholds(sT, tables);
     __syncthreads();
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
@@ -2098,11 +2104,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
     const lps_sphere* spheres = spheres0 + B.info.sphere0;
     const lps_plane* planes = planes0 + B.info.plane0;
     const float tf = xi.w;
-    // accepted pairs have d < 2^40; the predictions' coordinates were checked
-    // when they were written; the common case also takes the table index as
-    // the quotient d / div (one instantiation of the stencil per case, so the
-    // pair loop carries no mode test)
-    const bool fastdiv = c.s_max < 0x1p78f;
     float pfx = 0.f, pfy = 0.f, pfz = 0.f;
     auto pressure = [&](auto nocheck) {
         // the predicted-density pass of this correction recorded the masks
@@ -2121,7 +2122,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
             pfz += uz * sc * w;
         });
     };
-    if (fastdiv && g->tiny_pred == 0u && index_q_ok(c.div))
+    if (quick)
         pressure(std::true_type{});
     else
         pressure(std::false_type{});
@@ -2230,9 +2231,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure_q(
     const BlockScene B = block_scene(S, lb);
     const uint32_t stop = S.grid[B.s].stop_iter;
     if (stop != 0u && stop < k) return;
+    const bool fastdiv = c0.s_max < 0x1p78f;
+    const bool nocheck = fastdiv && g->tiny_pred == 0u;
+    const bool qidx = index_q_ok(c0.div);
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
-    load_thresholds(sT, tables);
+    // the thresholds serve only the pairs without the quotient index
+    if (!(nocheck && qidx)) load_thresholds(sT, tables);
     __syncthreads();
     const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
     if (i >= B.info.slot0 + B.info.count) return;
@@ -2241,9 +2246,6 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure_q(
     const float4 xa = pos[i];
     const float4 xi = xq[i];
     const float tf = xi.w;
-    const bool fastdiv = c.s_max < 0x1p78f;
-    const bool nocheck = fastdiv && g->tiny_pred == 0u;
-    const bool qidx = index_q_ok(c.div);
     float acc[3] = {0.f, 0.f, 0.f};
     for_neighbors_quad<3>(G, cell_lb, xa.x, xa.y, xa.z, xq, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_REPLAY,
                           CompactFilter{}, [&](uint32_t j) { return i != j; }, [](uint32_t) { return 0; },
