@@ -67,14 +67,58 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
     return peers;
 }
 
-__device__ __forceinline__ void load_table(float* dst, const float* src) {
-    for (uint32_t k = threadIdx.x; k < LPS_TABLE_SIZE; k += blockDim.x) dst[k] = src[k];
+// LDS copies of the kernel tables, staged by a workgroup of NT threads.  A
+// thread loads all of its entries of every table it stages before it
+// stores any, so the workgroup waits out one memory round trip; a strided
+// copy loop (the thread count unknown to the compiler) waits once per pass
+// and table.
+template <uint32_t NT, uint32_t N>
+struct TableSlice {
+    static constexpr uint32_t IT = (N + NT - 1) / NT;
+    float v[IT];
+    __device__ __forceinline__ void load(const float* __restrict__ src) {
+#pragma unroll
+        for (uint32_t r = 0; r < IT; r++) {
+            const uint32_t k = threadIdx.x + r * NT;
+            v[r] = k < N ? src[k] : 0.f;
+        }
+    }
+    __device__ __forceinline__ void store(float* dst) const {
+#pragma unroll
+        for (uint32_t r = 0; r < IT; r++) {
+            const uint32_t k = threadIdx.x + r * NT;
+            if (k < N) dst[k] = v[r];
+        }
+    }
+};
+
+// One table (at src) and, when with_thr, the index thresholds (stored after
+// the five tables).
+template <uint32_t NT>
+__device__ __forceinline__ void stage_table(float* dst, const float* src, float* thr, const float* tables,
+                                            bool with_thr = true) {
+    TableSlice<NT, LPS_TABLE_SIZE> a;
+    TableSlice<NT, LPS_THR_SIZE> t;
+    a.load(src);
+    if (with_thr) t.load(tables + 5 * LPS_TABLE_SIZE);
+    a.store(dst);
+    if (with_thr) t.store(thr);
 }
 
-// index thresholds, stored after the five tables
-__device__ __forceinline__ void load_thresholds(float* dst, const float* tables) {
-    const float* src = tables + 5 * LPS_TABLE_SIZE;
-    for (uint32_t k = threadIdx.x; k < LPS_THR_SIZE; k += blockDim.x) dst[k] = src[k];
+// The forces' three tables (W', W'' and the viscosity Laplacian) and the
+// thresholds.
+template <uint32_t NT>
+__device__ __forceinline__ void stage_force_tables(float* w1, float* w2, float* v2, float* thr, const float* tables) {
+    TableSlice<NT, LPS_TABLE_SIZE> a, b, c;
+    TableSlice<NT, LPS_THR_SIZE> t;
+    a.load(tables + 1 * LPS_TABLE_SIZE);
+    b.load(tables + 2 * LPS_TABLE_SIZE);
+    c.load(tables + 4 * LPS_TABLE_SIZE);
+    t.load(tables + 5 * LPS_TABLE_SIZE);
+    a.store(w1);
+    b.store(w2);
+    c.store(v2);
+    t.store(thr);
 }
 
 // ---------------------------------------------------- grid traversal --
@@ -1261,8 +1305,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sW, tables);
-    load_thresholds(sT, tables);
+    stage_table<LPS_BLOCK>(sW, tables, sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     const BlockScene B = block_scene(S, lb);
@@ -1305,10 +1348,7 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(FEAT 
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sW1, tables + 1 * LPS_TABLE_SIZE);
-    load_table(sW2, tables + 2 * LPS_TABLE_SIZE);
-    load_table(sV2, tables + 4 * LPS_TABLE_SIZE);
-    load_thresholds(sT, tables);
+    stage_force_tables<LPS_BLOCK>(sW1, sW2, sV2, sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     const BlockScene B = block_scene(S, lb);
@@ -1412,8 +1452,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph(const float4* __rest
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
-    load_thresholds(sT, tables);
+    stage_table<LPS_BLOCK>(sP1, tables + 3 * LPS_TABLE_SIZE, sT, tables);
     __syncthreads();
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
     const BlockScene B = block_scene(S, lb);
@@ -1636,8 +1675,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density_q(const float4* __restric
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sW, tables);
-    load_thresholds(sT, tables);
+    stage_table<LPS_BLOCK>(sW, tables, sT, tables);
     __syncthreads();
     const uint32_t i = quad_particle(lb);
     const BlockScene B = block_scene(S, i / LPS_BLOCK);
@@ -1676,10 +1714,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces_q(
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sW1, tables + 1 * LPS_TABLE_SIZE);
-    load_table(sW2, tables + 2 * LPS_TABLE_SIZE);
-    load_table(sV2, tables + 4 * LPS_TABLE_SIZE);
-    load_thresholds(sT, tables);
+    stage_force_tables<LPS_BLOCK>(sW1, sW2, sV2, sT, tables);
     __syncthreads();
     const uint32_t i = quad_particle(lb);
     const BlockScene B = block_scene(S, i / LPS_BLOCK);
@@ -1788,8 +1823,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph_q(const float4* __re
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
-    load_thresholds(sT, tables);
+    stage_table<LPS_BLOCK>(sP1, tables + 3 * LPS_TABLE_SIZE, sT, tables);
     __syncthreads();
     const uint32_t i = quad_particle(lb);
     const BlockScene B = block_scene(S, i / LPS_BLOCK);
@@ -1855,8 +1889,7 @@ __global__ __launch_bounds__(WG) void k_pci_density(
     const float4 xi = xpred[i];
     const uint2 qi = mk.cq[i];
     const BlockScene B = block_scene(S, lb);
-    load_table(sW, tables);
-    load_thresholds(sT, tables);
+    stage_table<WG>(sW, tables, sT, tables);
     __syncthreads();
     uint32_t ebits = 0;
     if (active && i < B.info.slot0 + B.info.count) {
@@ -2094,8 +2127,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(WG >= 128 ? 
     const bool fastdiv = c0.s_max < 0x1p78f;
     const bool quick = fastdiv && g->tiny_pred == 0u && index_q_ok(c0.div);
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
-    if (!quick) load_thresholds(sT, tables);
+    stage_table<WG>(sP1, tables + 3 * LPS_TABLE_SIZE, sT, tables, !quick);
     __syncthreads();
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
@@ -2156,8 +2188,7 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density_q(
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
     const BlockScene B = block_scene(S, lb);
-    load_table(sW, tables);
-    load_thresholds(sT, tables);
+    stage_table<LPS_BLOCK>(sW, tables, sT, tables);
     __syncthreads();
     const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
     uint32_t ebits = 0;
@@ -2235,9 +2266,8 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure_q(
     const bool nocheck = fastdiv && g->tiny_pred == 0u;
     const bool qidx = index_q_ok(c0.div);
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
-    load_table(sP1, tables + 3 * LPS_TABLE_SIZE);
     // the thresholds serve only the pairs without the quotient index
-    if (!(nocheck && qidx)) load_thresholds(sT, tables);
+    stage_table<LPS_BLOCK>(sP1, tables + 3 * LPS_TABLE_SIZE, sT, tables, !(nocheck && qidx));
     __syncthreads();
     const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
     if (i >= B.info.slot0 + B.info.count) return;
