@@ -842,6 +842,9 @@ __device__ __forceinline__ void scene_energy_block(const lps_dev::Scenes& S, con
     const SceneInfo si = S.info[sidx];
     const uint32_t nparts = si.padded / LPS_BLOCK;
     double ke = 0.0, pe = 0.0;
+    // unrolled so that several records' loads are in flight at once (the
+    // sums keep their order)
+#pragma unroll 8
     for (uint32_t k = threadIdx.x; k < nparts; k += LPS_BLOCK) {
         const uint32_t* q = part + (si.block0 + k) * LPS_BBOX_PART;
         ke += (double)__uint_as_float(q[7]);
@@ -886,6 +889,7 @@ __device__ __forceinline__ void scene_grid_block(const lps_dev::Scenes& S, const
         r[3 + a] = f2o(FLT_MIN);
     }
     r[6] = 0u;
+#pragma unroll 8
     for (uint32_t k = threadIdx.x; k < nparts; k += LPS_BLOCK) {
         const uint32_t* q = part + (si.block0 + k) * LPS_BBOX_PART;
         for (int a = 0; a < 3; a++) {
