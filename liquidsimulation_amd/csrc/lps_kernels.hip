@@ -1308,19 +1308,22 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density(const float4* __restrict_
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    // the slot's own state loads while the tables are staged (every slot of
+    // a block lies within the allocated slots); the reorder wrote the compact
+    // copy of these positions
+    const float4 xi = pos[i];
+    const uint2 qi = mk.cq[i];
+    const BlockScene B = block_scene(S, lb);
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     stage_table<LPS_BLOCK>(sW, tables, sT, tables);
     __syncthreads();
-    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    const BlockScene B = block_scene(S, lb);
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
-    const float4 xi = pos[i];
     float sum = 0.f;
-    // the reorder wrote the compact copy of these positions
     const float qs = compact_scale(G);
     CompactFilter cf{};
-    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
+    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, qi, compact_bias(c.s_max, qs)};
     for_neighbors_masked<2>(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                          cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
                          predicated([&](bool ok, uint32_t, const float4&, float s2, int, const float3&) {
@@ -1351,19 +1354,20 @@ __global__ __launch_bounds__(LPS_BLOCK) __attribute__((amdgpu_waves_per_eu(FEAT 
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    // the slot's own state loads while the tables are staged
+    const float4 xi = posrho[i];
+    const float4 vi = vel[i];
+    const BlockScene B = block_scene(S, lb);
     __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     stage_force_tables<LPS_BLOCK>(sW1, sW2, sV2, sT, tables);
     __syncthreads();
-    const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
-    const BlockScene B = block_scene(S, lb);
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
     const SimConst c = scene_const(c0, B.info);
     const lps_box* boxes = boxes0 + B.info.box0;
     const lps_sphere* spheres = spheres0 + B.info.sphere0;
     const lps_plane* planes = planes0 + B.info.plane0;
-    const float4 xi = posrho[i];
-    const float4 vi = vel[i];
     const float ri = xi.w;
     float fx = c.gravity[0] * ri, fy = c.gravity[1] * ri, fz = c.gravity[2] * ri;
     float snx = 0.f, sny = 0.f, snz = 0.f, lc = 0.f, vfx = 0.f, vfy = 0.f, vfz = 0.f;
@@ -2329,6 +2333,13 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     if (lb == UINT_MAX) return;
     const bool dead = step_dead(g);
     const uint32_t i = lb * LPS_BLOCK + threadIdx.x;
+    // the slot's own state loads before its scene's constants arrive (every
+    // slot of a block lies within the allocated slots)
+    const float4 x4 = pos[i];
+    const float4 f4 = force[i];
+    const float rho = posrho[i].w;
+    const float4 h4 = vh[i];
+    const bool is_first = first[i] != 0;
     const BlockScene B = block_scene(S, lb);
     const bool valid = i < B.info.slot0 + B.info.count;
     const SimConst c = scene_const(c0, B.info);
@@ -2338,22 +2349,19 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_integrate(
     float3 p = make_float3(0.f, 0.f, 0.f);
     float ke = 0.f, pe = 0.f;
     if (valid) {
-        const float4 x4 = pos[i];
         p = make_float3(x4.x, x4.y, x4.z);
         if (!dead) {
             // PCISPH: the pressure force and pressure of the buffer the
             // scene's loop chose; SPH: no pressure force here, buffer 0
             const size_t ob = fp ? (size_t)S.grid[B.s].final_buf * stride : 0u;
-            float4 f = force[i];
+            float4 f = f4;
             if (fp) {
                 const float4 a = fp[ob + i];
                 f.x += a.x; f.y += a.y; f.z += a.z;
             }
-            const float rho = posrho[i].w;
             const float ax = f.x / rho, ay = f.y / rho, az = f.z / rho;
-            const float4 h4 = vh[i];
             float3 hv = make_float3(h4.x, h4.y, h4.z);
-            if (first[i]) {
+            if (is_first) {
                 const float4 v4 = vel[i];
                 hv.x = v4.x - ax * c.dt / 2.f;
                 hv.y = v4.y - ay * c.dt / 2.f;
