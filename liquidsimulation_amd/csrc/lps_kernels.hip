@@ -1026,18 +1026,31 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_keys(const float4* __restrict__ p
         const uint32_t i = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
         p[r] = i < n ? pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    // the scene and grid of each of the tile's particle blocks (slot i lies
+    // in block i / LPS_BLOCK, uniform over a round), all loaded before the
+    // first key is formed
+    // (blocks past the last one read the last one's, unused)
+    uint32_t sidx[LPS_RADIX_ITEMS], kend[LPS_RADIX_ITEMS], kempty[LPS_RADIX_ITEMS];
+    GridView Gr[LPS_RADIX_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++)
+        sidx[r] = S.block_scene[min((tile * LPS_RADIX_TILE + r * LPS_BLOCK) / LPS_BLOCK, S.nblocks - 1u)];
+#pragma unroll
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const SceneGrid* sg = S.grid + sidx[r];
+        kend[r] = S.info[sidx[r]].slot0 + S.info[sidx[r]].count;
+        kempty[r] = sg->cell_base + sg->cells;
+        Gr[r] = grid_view(g, sg);
+    }
 #pragma unroll
     for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
         const uint32_t i = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
         const bool valid = i < n;
         uint32_t key = 0u;
         if (valid) {
-            // slot i lies in particle block i / LPS_BLOCK, uniform over the round
-            const BlockScene B = block_scene(S, i / LPS_BLOCK);
-            const SceneGrid* sg = S.grid + B.s;
-            key = sg->cell_base + sg->cells;
-            if (i < B.info.slot0 + B.info.count) {
-                const GridView G = grid_view(g, sg);
+            key = kempty[r];
+            if (i < kend[r]) {
+                const GridView& G = Gr[r];
                 const uint32_t xg = (uint32_t)truncf((p[r].x + G.off0) / G.gs);
                 const uint32_t yg = (uint32_t)truncf((p[r].y + G.off1) / G.gs);
                 const uint32_t zg = (uint32_t)truncf((p[r].z + G.off2) / G.gs);
