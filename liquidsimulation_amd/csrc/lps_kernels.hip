@@ -1077,13 +1077,21 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_radix_hist(const uint32_t* __rest
                                                           uint32_t* __restrict__ hist, uint32_t ntiles) {
     if (step_dead(g) || shift >= g->key_bits) return;
     __shared__ uint32_t cnt[256];
+    const uint32_t tile = blockIdx.x;
+    // the tile's keys load before the counters are cleared
+    uint32_t kr[LPS_RADIX_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const uint32_t idx = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
+        kr[r] = idx < n ? keys[idx] : 0u;
+    }
     cnt[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t tile = blockIdx.x;
+#pragma unroll
     for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
         const uint32_t idx = tile * LPS_RADIX_TILE + r * LPS_BLOCK + threadIdx.x;
         const bool valid = idx < n;
-        const uint32_t d = valid ? (keys[idx] >> shift) & 0xFFu : 0u;
+        const uint32_t d = valid ? (kr[r] >> shift) & 0xFFu : 0u;
         const uint64_t act = __ballot(valid);
         const uint64_t peers = match_digit(d, act);
         if (valid && lane_id() == (uint32_t)__ffsll((long long)peers) - 1u) atomicAdd(&cnt[d], (uint32_t)__popcll(peers));
@@ -1143,9 +1151,19 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_radix_scatter(const uint32_t* __r
     __shared__ uint32_t s_ws[LPS_BLOCK / 64];
     const uint32_t t = threadIdx.x, w = t >> 6, lane = lane_id();
     const uint32_t tile = blockIdx.x;
+    // every load of the block issues before its first barrier: the digit
+    // totals, this tile's offsets and its items
+    const uint32_t v = digit_total[t];
+    const uint32_t tile_off = hist[t * ntiles + tile];
+    uint32_t kr[LPS_RADIX_ITEMS], vr[LPS_RADIX_ITEMS];
+#pragma unroll
+    for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
+        const uint32_t idx = tile * LPS_RADIX_TILE + r * LPS_BLOCK + t;
+        kr[r] = idx < n ? kin[idx] : 0u;
+        vr[r] = idx < n ? vin[idx] : 0u;
+    }
     {
         // exclusive scan of the 256 digit totals
-        const uint32_t v = digit_total[t];
         uint32_t incl = v;
         for (int off = 1; off < 64; off <<= 1) {
             const uint32_t y = (uint32_t)__shfl_up((int)incl, off, 64);
@@ -1155,19 +1173,16 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_radix_scatter(const uint32_t* __r
         __syncthreads();
         uint32_t wpre = 0;
         for (uint32_t q = 0; q < w; q++) wpre += s_ws[q];
-        s_off[t] = hist[t * ntiles + tile] + wpre + incl - v;
+        s_off[t] = tile_off + wpre + incl - v;
     }
     for (uint32_t k = 0; k < LPS_BLOCK / 64; k++) s_w[k][t] = 0u;
     __syncthreads();
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+#pragma unroll
     for (uint32_t r = 0; r < LPS_RADIX_ITEMS; r++) {
         const uint32_t idx = tile * LPS_RADIX_TILE + r * LPS_BLOCK + t;
         const bool valid = idx < n;
-        uint32_t key = 0, val = 0;
-        if (valid) {
-            key = kin[idx];
-            val = vin[idx];
-        }
+        const uint32_t key = kr[r], val = vr[r];
         const uint32_t d = (key >> shift) & 0xFFu;
         const uint64_t act = __ballot(valid);
         const uint64_t peers = match_digit(d, act);
