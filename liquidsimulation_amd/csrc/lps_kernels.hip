@@ -1710,17 +1710,20 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_density_q(const float4* __restric
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    // the slot's own state loads while the tables are staged (every slot of
+    // a block lies within the allocated slots)
+    const uint32_t i = quad_particle(lb);
+    const float4 xi = pos[i];
+    const uint2 qi = mk.cq[i];
+    const BlockScene B = block_scene(S, i / LPS_BLOCK);
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     stage_table<LPS_BLOCK>(sW, tables, sT, tables);
     __syncthreads();
-    const uint32_t i = quad_particle(lb);
-    const BlockScene B = block_scene(S, i / LPS_BLOCK);
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
-    const float4 xi = pos[i];
     const float qs = compact_scale(G);
     CompactFilter cf{};
-    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
+    if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, qi, compact_bias(c.s_max, qs)};
     float acc[1] = {0.f};
     for_neighbors_quad<1>(G, cell_lb, xi.x, xi.y, xi.z, pos, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_RECORD,
                           cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
@@ -1749,16 +1752,17 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_forces_q(
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    // the slot's own state loads while the tables are staged
+    const uint32_t i = quad_particle(lb);
+    const float4 xi = posrho[i];
+    const float4 vi = vel[i];
+    const BlockScene B = block_scene(S, i / LPS_BLOCK);
     __shared__ float sW1[LPS_TABLE_SIZE], sW2[LPS_TABLE_SIZE], sV2[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     stage_force_tables<LPS_BLOCK>(sW1, sW2, sV2, sT, tables);
     __syncthreads();
-    const uint32_t i = quad_particle(lb);
-    const BlockScene B = block_scene(S, i / LPS_BLOCK);
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
     const SimConst c = scene_const(c0, B.info);
-    const float4 xi = posrho[i];
-    const float4 vi = vel[i];
     const float ri = xi.w;
     // snx, sny, snz, lc, vfx, vfy, vfz, then (features) nb, dv
     constexpr int NT = FEAT ? 9 : 7;
@@ -1858,15 +1862,16 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pressure_sph_q(const float4* __re
     if (step_dead(g)) return;
     const uint32_t lb = xcd_block(nlogical);
     if (lb == UINT_MAX) return;
+    // the slot's own state loads while the tables are staged
+    const uint32_t i = quad_particle(lb);
+    const float4 xi = posrho[i];
+    const float tf = q[i];
+    const BlockScene B = block_scene(S, i / LPS_BLOCK);
     __shared__ float sP1[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     stage_table<LPS_BLOCK>(sP1, tables + 3 * LPS_TABLE_SIZE, sT, tables);
     __syncthreads();
-    const uint32_t i = quad_particle(lb);
-    const BlockScene B = block_scene(S, i / LPS_BLOCK);
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
-    const float4 xi = posrho[i];
-    const float tf = q[i];
     const bool fastdiv = c.s_max < 0x1p78f;  // accepted pairs have d < 2^40
     const bool qidx = index_q_ok(c.div);
     float acc[3] = {0.f, 0.f, 0.f};
@@ -2223,18 +2228,21 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_density_q(
     if (lb == UINT_MAX) return;
     __shared__ float sW[LPS_TABLE_SIZE], sT[LPS_THR_SIZE];
     __shared__ uint32_t s_max[LPS_BLOCK / 64];
+    // the slot's own state loads while the tables are staged (every slot of
+    // an active piece lies within the allocated slots)
+    const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
+    const float4 xa = pos[i];
+    const float4 xi = xpred[i];
+    const uint2 qi = mk.cq[i];
     const BlockScene B = block_scene(S, lb);
     stage_table<LPS_BLOCK>(sW, tables, sT, tables);
     __syncthreads();
-    const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
     uint32_t ebits = 0;
     if (i < B.info.slot0 + B.info.count) {
         const GridView G = grid_view(g, S.grid + B.s);
-        const float4 xa = pos[i];
-        const float4 xi = xpred[i];
         const float qs = compact_scale(G);
         CompactFilter cf{};
-        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, mk.cq[i], compact_bias(c.s_max, qs)};
+        if (compact_near(xi.x, xi.y, xi.z, G, qs)) cf = CompactFilter{mk.cq, qi, compact_bias(c.s_max, qs)};
         float acc[1] = {0.f};
         for_neighbors_quad<1>(G, cell_lb, xa.x, xa.y, xa.z, xpred, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i,
                               MC_RECORD, cf, [](uint32_t) { return true; }, [](uint32_t) { return 0; },
@@ -2295,6 +2303,11 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure_q(
     uint32_t s0 = 0u;
     const uint32_t lb = active_piece<LPS_BLOCK / LPS_QUAD>(g, S, g->list_of[k % LPS_MAX_ITER_SLOTS], s0);
     if (lb == UINT_MAX) return;
+    // the slot's own state loads while the tables are staged; the
+    // predicted-density pass's (prediction, p / rho*^2)
+    const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
+    const float4 xa = pos[i];
+    const float4 xi = xq[i];
     const BlockScene B = block_scene(S, lb);
     const uint32_t stop = S.grid[B.s].stop_iter;
     if (stop != 0u && stop < k) return;
@@ -2305,12 +2318,9 @@ __global__ __launch_bounds__(LPS_BLOCK) void k_pci_pressure_q(
     // the thresholds serve only the pairs without the quotient index
     stage_table<LPS_BLOCK>(sP1, tables + 3 * LPS_TABLE_SIZE, sT, tables, !(nocheck && qidx));
     __syncthreads();
-    const uint32_t i = s0 + threadIdx.x / LPS_QUAD;
     if (i >= B.info.slot0 + B.info.count) return;
     const GridView G = grid_view(g, S.grid + B.s);
     const SimConst c = scene_const(c0, B.info);
-    const float4 xa = pos[i];
-    const float4 xi = xq[i];
     const float tf = xi.w;
     float acc[3] = {0.f, 0.f, 0.f};
     for_neighbors_quad<3>(G, cell_lb, xa.x, xa.y, xa.z, xq, xi.x, xi.y, xi.z, c.s_max, c.s_relax, mk, i, MC_REPLAY,
